@@ -1,0 +1,199 @@
+"""Write the golden vectors under tests/golden/ from the compiled reference.
+
+Run in a container that has the reference checkout (its sources are compiled
+by `make -C oracle ref` into oracle/_ref/); the GPU box only reads the
+fixtures this script writes.
+
+Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
+  gpp_<case>.npz   inputs + achtemp/asxtemp/acht_n1_loc + per-column
+                   scht/ssxt sums written by oracle/_ref/ref_gpp_harness,
+                   i.e. by the reference's flagOCC_solver/ssxt_scht_solver
+                   (BGW-Kernels/gppKer.cpp:19-65,130-159)
+  gpp_points.npz   per-point ssx/sch of the reference's ssxt_scht_solver
+  gpp_main.npz     achtemp printed by the reference's own gppKer program
+                   (BGW-Kernels/gppKer.cpp main, constant inputs) at small
+                   argv shapes, OMP_NUM_THREADS=1
+  ff_achs.npz      achsDtemp of the reference's achsDtemp_Kernel
+                   (FFKer_cori/ffKernel.cpp:107-131, frequency index 1)
+  ff_main.npz      achsDtemp printed by the reference's ffKernel program
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import fko  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def run_gpp_harness(p: fko.GppProblem):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([p.nb, p.nv, p.ncouls, p.ngpown, p.nw], np.int32).tofile(f)
+            np.array([p.occ], np.float64).tofile(f)
+            np.ascontiguousarray(p.wx, np.float64).tofile(f)
+            for a in (p.aqsm, p.aqsn, p.I_eps, p.wtilde):
+                np.ascontiguousarray(a, np.complex128).tofile(f)
+            np.ascontiguousarray(p.vcoul, np.float64).tofile(f)
+            np.ascontiguousarray(p.inv_igp_index, np.int32).tofile(f)
+            np.ascontiguousarray(p.indinv, np.int32).tofile(f)
+        subprocess.run([os.path.join(REF_BIN, "ref_gpp_harness"), fin, fout], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        out = np.fromfile(fout, np.complex128)
+    nw, nb, G = p.nw, p.nb, p.ngpown
+    o = 0
+    ach = out[o:o + nw]; o += nw
+    asx = out[o:o + nw]; o += nw
+    n1loc = out[o:o + nb]; o += nb
+    scht = out[o:o + nb * G * nw].reshape(nb, G, nw); o += nb * G * nw
+    ssxt = out[o:o + nb * G * nw].reshape(nb, G, nw); o += nb * G * nw
+    assert o == out.size
+    return ach, asx, n1loc, scht, ssxt
+
+
+def save_gpp(name: str, p: fko.GppProblem, **extra):
+    ach, asx, n1loc, scht, ssxt = run_gpp_harness(p)
+    np.savez_compressed(
+        os.path.join(HERE, f"gpp_{name}.npz"),
+        nb=p.nb, nv=p.nv, ncouls=p.ncouls, ngpown=p.ngpown, occ=p.occ,
+        wx=p.wx, aqsm=p.aqsm, aqsn=p.aqsn, I_eps=p.I_eps, wtilde=p.wtilde, vcoul=p.vcoul,
+        inv_igp_index=p.inv_igp_index, indinv=p.indinv,
+        achtemp=ach, asxtemp=asx, acht_n1_loc=n1loc, scht=scht, ssxt=ssxt, **extra)
+    print(f"gpp_{name}: achtemp={ach} asxtemp={asx}")
+
+
+def case_random():
+    return fko.gpp_random(6, 2, 96, 7, nw=3, seed=11)
+
+
+def case_nearpole():
+    """wtilde.re ~ U(2.5,5.5) puts points within 0.5 of wx = 3,4,5 (branch 2),
+    plus points within |wt|/500 of a pole (|delw|^2 >= limitone)."""
+    p = fko.gpp_random(5, 3, 80, 6, nw=3, seed=12, wtilde_re=(2.5, 5.5))
+    w = p.wtilde.copy()
+    w[0, 3] = 3.0 + 1e-4 + 1e-5j
+    w[1, 7] = 4.0 - 2e-3 + 0.0j
+    w[2, 11] = 5.0 + 5e-3 - 1e-3j
+    w[3, 0] = 3.2 + 0.4j
+    p.wtilde = w
+    return p
+
+
+def case_branch3():
+    """Small wx so that |wdiff|^2 <= limittwo with |delw|^2 <= to1 (branch 3)."""
+    p = fko.gpp_random(4, 4, 64, 5, nw=3, seed=13, wtilde_re=(-0.05, 0.3))
+    p.wx = np.array([0.2, 0.35, 3.0])
+    w = p.wtilde.copy()
+    rng = np.random.default_rng(13)
+    idx = rng.choice(w.size, 60, replace=False)
+    w.flat[idx] = (rng.uniform(-2e-4, 2e-4, 60) + 1j * rng.uniform(-2e-4, 2e-4, 60))
+    p.wtilde = w
+    return p
+
+
+def case_skipmap():
+    """C++ map (i+1)*C/G with the indigp == C clamp, plus skipped columns
+    (negative indinv entries, BGW-Kernels/gppKer.cpp:322-328)."""
+    p = fko.gpp_random(4, 1, 64, 9, nw=4, seed=14, map_kind="cpp")
+    ind = p.indinv.copy()
+    ind[p.inv_igp_index[2]] = -1
+    ind[p.inv_igp_index[5]] = -7
+    p.indinv = ind
+    return p
+
+
+def points():
+    """Per-point golden values: nb=1, C=1, aqsm=aqsn=1 so each column sum is the
+    point's ssx/sch exactly (flagOCC_solver with igmax=1)."""
+    rng = np.random.default_rng(21)
+    K = 40
+    wt = rng.uniform(-1, 6, K) + 1j * rng.uniform(-0.3, 0.3, K)
+    wt[:8] = np.array([3 + 1e-4j, 4.0001, 5 - 3e-3 + 1e-3j, 1e-4 + 1e-4j, 2.99, 3.3 + 0.1j, 0.19 + 0j, 4.2 - 0.2j])
+    ie = rng.uniform(-0.5, 0.5, K) + 1j * rng.uniform(-0.5, 0.5, K)
+    wx = np.array([0.2, 3.0, 4.0, 5.0, 1e-6])
+    p = fko.GppProblem(1, 1, 1, K, wx, np.ones((1, 1), complex), np.ones((1, 1), complex),
+                       ie.reshape(K, 1), wt.reshape(K, 1), np.ones(1), np.zeros(K, np.int32),
+                       np.array([0, 0], np.int32))
+    _, _, _, scht, ssxt = run_gpp_harness(p)
+    np.savez_compressed(os.path.join(HERE, "gpp_points.npz"), wx=wx, wtilde=wt, I_eps=ie,
+                        ssx=ssxt[0], sch=scht[0])
+    print("gpp_points: written")
+
+
+def reference_main():
+    shapes = [(8, 2, 1024, 20), (16, 4, 512, 8), (6, 6, 256, 5)]
+    rows = []
+    for nb, nv, C, npg in shapes:
+        out = subprocess.run([os.path.join(REF_BIN, "gppKer"), str(nb), str(nv), str(C), str(npg)],
+                             check=True, capture_output=True, text=True,
+                             env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
+        vals = re.findall(r"achtemp\[(\d)\] = \(([-\d.e+]+),([-\d.e+]+)\)", out)
+        assert len(vals) == 3, out
+        rows.append([complex(float(a), float(b)) for _, a, b in vals])
+    np.savez_compressed(os.path.join(HERE, "gpp_main.npz"), shapes=np.array(shapes, np.int64),
+                        achtemp=np.array(rows))
+    print("gpp_main:", rows)
+
+
+def run_ff_harness(p: fko.FfProblem):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([p.nb, p.ncouls, p.ngpown, p.nfreq], np.int32).tofile(f)
+            for a in (p.aqsm, p.aqsn, p.I_eps):
+                np.ascontiguousarray(a, np.complex128).tofile(f)
+            np.ascontiguousarray(p.vcoul, np.float64).tofile(f)
+            np.ascontiguousarray(p.inv_igp_index, np.int32).tofile(f)
+            np.ascontiguousarray(p.indinv[: p.ncouls], np.int32).tofile(f)
+        subprocess.run([os.path.join(REF_BIN, "ref_ff_harness"), fin, fout], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        return complex(*np.fromfile(fout, np.float64))
+
+
+def ff_cases():
+    p = fko.ff_random(5, 48, 6, 3, seed=31)
+    achs = run_ff_harness(p)
+    np.savez_compressed(os.path.join(HERE, "ff_achs.npz"), nb=p.nb, ncouls=p.ncouls, ngpown=p.ngpown,
+                        nfreq=p.nfreq, aqsm=p.aqsm, aqsn=p.aqsn, I_eps=p.I_eps, vcoul=p.vcoul,
+                        inv_igp_index=p.inv_igp_index, indinv=p.indinv, achsDtemp=achs)
+    print("ff_achs:", achs)
+    shapes = [(8, 2, 64, 32, 4, 3), (4, 2, 32, 16, 3, 2)]
+    rows = []
+    for s in shapes:
+        out = subprocess.run([os.path.join(REF_BIN, "ffKernel")] + [str(v) for v in s], check=True,
+                             capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
+        m = re.search(r"achsDtemp = \( ([-\d.]+), ([-\d.]+)\)", out)
+        assert m, out
+        rows.append(complex(float(m.group(1)), float(m.group(2))))
+    np.savez_compressed(os.path.join(HERE, "ff_main.npz"), shapes=np.array(shapes, np.int64),
+                        achsDtemp=np.array(rows))
+    print("ff_main:", rows)
+
+
+def main():
+    for exe in ("ref_gpp_harness", "ref_ff_harness", "gppKer", "ffKernel"):
+        if not os.path.exists(os.path.join(REF_BIN, exe)):
+            sys.exit(f"missing oracle/_ref/{exe}: run `make -C oracle ref` where the reference is present")
+    save_gpp("random", case_random())
+    save_gpp("nearpole", case_nearpole())
+    save_gpp("branch3", case_branch3())
+    save_gpp("skipmap", case_skipmap())
+    points()
+    reference_main()
+    ff_cases()
+
+
+if __name__ == "__main__":
+    main()
