@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPP band sum on one MI355X.
+
+A step is one gpp_kernel call (FP64, device memory, inputs resident in HBM)
+over the 512/2/32768/20 workload: number_bands 512, nvband 2, ncouls 32768,
+ngpown 1638, 3 frequencies, seeded synthetic inputs (aqs*/I_eps ~ U(-0.5,0.5)
+complex, wtilde re ~ U(0.5,2), im ~ U(-0.01,0.01), vcoul ~ U(0.1,10)).
+
+Prints ONE JSON line on stdout.  value = updates/s (n1 x igp x ig x iw per
+second).  roofline = the fused GEMM+epilogue kernel's algorithmic FP64 flops
+over its HIP-event-timed duration against the 78.6 TFLOP/s FP64 MFMA peak.
+cpu_baseline = the reference's own gppKer program (oracle/_ref, built from
+the reference sources) on a bounded band sample of the same shape, timed on
+this host's cores; the CPU oracle restatement is used if it is absent.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GPP updates/s (n1·igp·ig·iw) + % FP64/FP32 vector peak, FP64 vs mixed"
+PEAK_FP64 = 78.6      # TFLOP/s, MI355X FP64 vector = FP64 MFMA (dense)
+PEAK_FP32 = 157.3     # TFLOP/s, FP32 vector = FP32 MFMA
+NB, NV, NCOULS, NPG, NW = 512, 2, 32768, 20, 3
+STREAMS = {"aqsm": 1, "aqsn": 2, "I_eps": 3, "wtilde": 4, "vcoul": 5}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gpp_alg_flops(nb, nv, C, G, nw):
+    """FP64 flops of the factorised evaluation (one reciprocal = 1 flop):
+    8 nb G C for B = sum_n1 c[n1] aqsn[n1] (complex MAC), and per (igp, ig):
+    8 nv (occupied-band contraction) + 9 (wtilde^2, Omega2) + nw x 42
+    (pole factor 22, SX factor 4, two complex MACs 16)."""
+    return 8.0 * nb * G * C + G * C * (8.0 * nv + 9.0 + 42.0 * nw)
+
+
+def gpp_reference_flops(nb, nv, C, G, nw):
+    """The per-update count of the unfactorised loop with band-invariant work
+    hoisted: 8U + 6 nb G C + 22 G C nw + 10 nv G C nw."""
+    U = nb * G * C * nw
+    return 8.0 * U + 6.0 * nb * G * C + 22.0 * G * C * nw + 10.0 * nv * G * C * nw
+
+
+def import_fk():
+    import importlib.util
+    pkg = os.path.join(ROOT, "fortkernels-_amd")
+    spec = importlib.util.spec_from_file_location("fortkernels_amd", os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["fortkernels_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def make_inputs(fk, torch, nb, C, G, seed, cdtype, wtilde_re):
+    dev = torch.device("cuda:0")
+    t = {}
+    for name, rows, re_rng, im_rng in (("aqsm", nb, (-0.5, 0.5), (-0.5, 0.5)), ("aqsn", nb, (-0.5, 0.5), (-0.5, 0.5)),
+                                       ("I_eps", G, (-0.5, 0.5), (-0.5, 0.5)),
+                                       ("wtilde", G, wtilde_re, (-0.01, 0.01))):
+        t[name] = torch.empty((rows, C), dtype=cdtype, device=dev)
+        fk.fill_complex(t[name], seed, STREAMS[name], re_rng, im_rng)
+    t["vcoul"] = torch.empty(C, dtype=torch.float64, device=dev)
+    fk.fill_real(t["vcoul"], seed, STREAMS["vcoul"], 0.1, 10.0)
+    torch.cuda.synchronize()
+    return t
+
+
+def index_maps(C, G):
+    inv = (np.arange(G, dtype=np.int64) * C // G).astype(np.int32)     # i*ncouls/ngpown
+    indinv = np.arange(C + 1, dtype=np.int32)
+    indinv[C] = C - 1
+    return inv, indinv
+
+
+def wx_array(nw):
+    return np.array([10.0 - 6.0 + 1.0 * ((iw + 1) - 2) for iw in range(nw)])
+
+
+def run_gpp(fk, torch, precision, steps, warmup, seed=1):
+    C, G = NCOULS, NCOULS // NPG
+    cdt = torch.complex128 if precision == "fp64" else torch.complex64
+    wre = (0.5, 2.0) if precision == "fp64" else (3.0, 5.0)
+    log(f"{precision}: generating inputs nb={NB} nv={NV} ncouls={C} ngpown={G}")
+    t = make_inputs(fk, torch, NB, C, G, seed, cdt, wre)
+    inv, indinv = index_maps(C, G)
+    wx = wx_array(NW)
+    nbytes = fk.gpp_workspace_size(NB, NV, C, G, NW, precision=precision)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def step(time_kernel=False):
+        return fk.gpp_kernel(NB, NV, C, G, wx, t["aqsm"], t["aqsn"], t["I_eps"], t["wtilde"], t["vcoul"],
+                             inv, indinv, precision=precision, stream=stream, workspace=ws,
+                             time_kernel=time_kernel)
+
+    log(f"{precision}: warmup {warmup}")
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"{precision}: timing {steps} steps")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        out = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1) / steps
+    ach = out[0].cpu().numpy()
+    # dominant kernel: per-launch duration from HIP events on the launch stream
+    kms = [step(time_kernel=True)[2] for _ in range(max(steps, 5))]
+    torch.cuda.synchronize()
+    return dict(ms_per_step=1e3 * wall / steps, event_ms_per_step=ev_ms, kernel_ms=float(np.mean(kms)),
+                kernel_ms_min=float(np.min(kms)), achtemp=ach, C=C, G=G)
+
+
+def cpu_baseline(bands, threads):
+    """The reference's own gppKer (oracle/_ref) on `bands` bands of the
+    512/2/32768/20 shape, OMP threads = `threads`."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "gppKer")
+    C, G = NCOULS, NCOULS // NPG
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores",
+               OMP_STACKSIZE="16M")
+    if os.access(exe, os.X_OK):
+        log(f"cpu baseline: reference gppKer {bands} {NV} {C} {NPG} on {threads} threads")
+        out = subprocess.run([exe, str(bands), str(NV), str(C), str(NPG)], env=env, capture_output=True,
+                             text=True, timeout=600, check=True).stdout
+        secs = float(re.search(r"Chrono Time Taken \*+= ([0-9.eE+-]+)", out).group(1))
+        kind = "reference"
+    else:
+        from oracle import fko
+        log(f"cpu baseline: oracle port, {bands} bands on {threads} threads")
+        p = fko.gpp_random(bands, NV, C, G, seed=1)
+        t0 = time.perf_counter()
+        fko.gpp(p, extended=False if threads == 1 else True, nthreads=threads)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    U = bands * G * C * NW
+    return {"value": U / secs, "unit": "updates/s", "cores": threads, "kind": kind,
+            "sample": f"{bands} of {NB} bands, nvband {NV}, ncouls {C}, ngpown {G}, 3 frequencies "
+                      f"({U:.3e} updates, {secs:.2f} s)"}
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-bands", type=int, default=128)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the GPP band sum is benchmarked per device (a multi-GPU run "
+              "would shard ngpown columns and all-reduce 2*nfreq values); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    fk = import_fk()
+    assert torch.cuda.is_available(), "no ROCm device"
+    C, G = NCOULS, NCOULS // NPG
+    U = NB * G * C * NW
+
+    r = run_gpp(fk, torch, "fp64", args.steps, args.warmup)
+    f_alg = gpp_alg_flops(NB, NV, C, G, NW)
+    achieved = f_alg / (r["kernel_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": U / (r["ms_per_step"] * 1e-3),
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "GPP band sum 512/2/32768/20 (ngpown 1638, 3 frequencies), FP64",
+                   "number_bands": NB, "nvband": NV, "ncouls": C, "ngpown": G, "nfreq": NW},
+        "roofline": {"bound": "mfma", "kernel": "gpp_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": load_traffic("gpp_fused_fp64_traffic.json"),
+                     "alg_flops_per_launch": f_alg, "kernel_ms": r["kernel_ms"]},
+        "reference_equivalent_tflops": gpp_reference_flops(NB, NV, C, G, NW) / (r["ms_per_step"] * 1e-3) / 1e12,
+        "event_ms_per_step": r["event_ms_per_step"],
+    }
+    log(f"fp64: {line['ms_per_step']:.3f} ms/step, kernel {r['kernel_ms']:.3f} ms, "
+        f"{achieved:.1f} TFLOP/s ({100 * achieved / PEAK_FP64:.1f}% of FP64 peak)")
+    if not args.no_mixed:
+        m = run_gpp(fk, torch, "mixed", max(5, args.steps // 2), args.warmup)
+        ach_m = f_alg / (m["kernel_ms"] * 1e-3) / 1e12
+        line["mixed"] = {"value": U / (m["ms_per_step"] * 1e-3), "ms_per_step": m["ms_per_step"],
+                         "kernel_ms": m["kernel_ms"], "speedup_vs_fp64": r["ms_per_step"] / m["ms_per_step"],
+                         "achieved_tflops": ach_m, "peak_fp32_tflops": PEAK_FP32,
+                         "workload": "same shape, complex64 storage, wtilde re ~ U(3,5)"}
+        log(f"mixed: {m['ms_per_step']:.3f} ms/step ({line['mixed']['speedup_vs_fp64']:.2f}x vs fp64)")
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_bands, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

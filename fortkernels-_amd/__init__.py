@@ -1,0 +1,279 @@
+"""Python binding of the MI355X GPP / FF library (include/fortkernels.h).
+
+The product is the C ABI in lib/libfortkernels_amd.so (HIP, gfx950); this
+module is a thin ctypes layer over it.  Arrays are numpy arrays (host
+memory: the library copies, runs and copies back) or torch tensors on a
+ROCm device (device memory: the call is asynchronous on the given stream).
+There is no fallback: if the shared library is missing, importing this
+module raises.
+
+The names mirror the reference's driver arguments (BGW-Kernels/gppKer.cpp
+main: number_bands, nvband, ncouls, nodes_per_group) and the C ABI entry
+points gpp_kernel / ff_kernel.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfortkernels_amd.so")
+INCLUDE = os.path.join(os.path.dirname(_HERE), "include", "fortkernels.h")
+
+FK_OK, FK_ERR_ARG, FK_ERR_INDEX, FK_ERR_UNSUPPORTED, FK_ERR_HIP, FK_ERR_NOMEM, FK_ERR_WORKSPACE = 0, -1, -2, -3, -4, -5, -6
+FK_MEM_HOST, FK_MEM_DEVICE = 0, 1
+FK_PREC_FP64, FK_PREC_MIXED = 0, 1
+COP = {"mul": 0, "conj": 1, "abs2": 2, "abs": 3, "scale": 4, "fma": 5, "fms": 6, "real_minus": 7,
+       "real_plus": 8, "div": 9, "div_real": 10, "square": 11, "add": 12, "sub": 13}
+
+# every extern "C" symbol of include/fortkernels.h
+EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_kernel",
+           "ff_kernel_workspace_size", "fk_last_error", "fk_abi_version", "fk_build_target",
+           "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe"]
+
+
+class FkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("memory", ctypes.c_int32), ("precision", ctypes.c_int32), ("mask", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("stream", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+                ("workspace_bytes", ctypes.c_size_t), ("config", ctypes.c_int32),
+                ("kernel_ms", ctypes.POINTER(ctypes.c_float))]
+
+
+class GppProblem(ctypes.Structure):
+    _fields_ = [("number_bands", ctypes.c_int32), ("nvband", ctypes.c_int32), ("ncouls", ctypes.c_int32),
+                ("ngpown", ctypes.c_int32), ("nfreq", ctypes.c_int32), ("wx", ctypes.c_void_p),
+                ("aqsmtemp", ctypes.c_void_p), ("aqsntemp", ctypes.c_void_p), ("I_eps_array", ctypes.c_void_p),
+                ("wtilde_array", ctypes.c_void_p), ("vcoul", ctypes.c_void_p),
+                ("inv_igp_index", ctypes.c_void_p), ("indinv", ctypes.c_void_p),
+                ("to1", ctypes.c_double), ("limittwo", ctypes.c_double), ("limitone", ctypes.c_double),
+                ("occ", ctypes.c_double)]
+
+
+class FfProblem(ctypes.Structure):
+    _fields_ = [("number_bands", ctypes.c_int32), ("ncouls", ctypes.c_int32), ("ngpown", ctypes.c_int32),
+                ("nfreq", ctypes.c_int32), ("aqsmtemp", ctypes.c_void_p), ("aqsntemp", ctypes.c_void_p),
+                ("I_eps_array", ctypes.c_void_p), ("vcoul", ctypes.c_void_p),
+                ("inv_igp_index", ctypes.c_void_p), ("indinv", ctypes.c_void_p), ("achs_freq", ctypes.c_int32)]
+
+
+def _load():
+    # torch (when installed) ships its own HIP runtime whose SONAME is the
+    # libamdhip64.so.7 this library needs.  Loading torch first makes the
+    # dynamic loader reuse that runtime for the library, so device pointers
+    # and streams from torch are valid here; loading the library first would
+    # put a second, separate HIP runtime into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"fortkernels-_amd: shared library not built: {LIB_PATH} "
+                          "(run `make -C fortkernels-_amd` or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I32, I64, U64, D, SZ = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
+                               ctypes.c_double, ctypes.c_size_t)
+    L.fk_default_options.restype = Options
+    L.fk_default_options.argtypes = []
+    L.gpp_kernel.restype = ctypes.c_int
+    L.gpp_kernel.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), P, P, P]
+    L.gpp_kernel_workspace_size.restype = SZ
+    L.gpp_kernel_workspace_size.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options)]
+    L.ff_kernel.restype = ctypes.c_int
+    L.ff_kernel.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options), P, P, P]
+    L.ff_kernel_workspace_size.restype = SZ
+    L.ff_kernel_workspace_size.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options)]
+    L.fk_last_error.restype = ctypes.c_char_p
+    L.fk_last_error.argtypes = []
+    L.fk_abi_version.restype = ctypes.c_int
+    L.fk_build_target.restype = ctypes.c_char_p
+    L.fk_fill_complex.restype = ctypes.c_int
+    L.fk_fill_complex.argtypes = [P, I64, U64, U64, D, D, D, D, I32, P]
+    L.fk_fill_real.restype = ctypes.c_int
+    L.fk_fill_real.argtypes = [P, I64, U64, U64, D, D, P]
+    L.fk_complex_apply.restype = ctypes.c_int
+    L.fk_complex_apply.argtypes = [I32, I64, P, P, P, P, I32]
+    L.fk_mfma_probe.restype = ctypes.c_int
+    L.fk_mfma_probe.argtypes = [I32, P]
+    return L
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    return lib.fk_last_error().decode()
+
+
+def _check(rc):
+    if rc != FK_OK:
+        raise FkError(rc, last_error())
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if _is_torch(x):
+        assert x.is_contiguous(), "tensor must be contiguous"
+        return x.data_ptr()
+    assert isinstance(x, np.ndarray) and x.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return x.ctypes.data
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream  # torch.cuda.Stream
+
+
+def default_options() -> Options:
+    return lib.fk_default_options()
+
+
+# --------------------------------------------------------------------------
+# GPP
+# --------------------------------------------------------------------------
+def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv_igp_index, indinv, *,
+               to1=1e-6, limittwo=0.25, limitone=2.5e5, occ=1.0, precision="fp64", mask=True,
+               stream=None, workspace=None, config=0, time_kernel=False, device=-1):
+    """Run gpp_kernel.  Returns (achtemp, asxtemp, kernel_ms|None).
+
+    numpy inputs -> host memory mode, outputs are numpy complex128 arrays.
+    torch device tensors -> device mode, outputs are torch complex128 device
+    tensors (valid once the stream has synchronised)."""
+    wx = np.ascontiguousarray(wx, np.float64)
+    inv = np.ascontiguousarray(inv_igp_index, np.int32)
+    ind = np.ascontiguousarray(indinv, np.int32)
+    nw = int(wx.shape[0])
+    on_dev = _is_torch(aqsn)
+    prob = GppProblem(int(nb), int(nv), int(ncouls), int(ngpown), nw, wx.ctypes.data, _ptr(aqsm), _ptr(aqsn),
+                      _ptr(I_eps), _ptr(wtilde), _ptr(vcoul), inv.ctypes.data, ind.ctypes.data,
+                      float(to1), float(limittwo), float(limitone), float(occ))
+    opt = default_options()
+    opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.mask = 1 if mask else 0
+    opt.device = int(device)
+    opt.stream = _stream_handle(stream)
+    if workspace is not None:
+        opt.workspace = _ptr(workspace)
+        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.config = int(config)
+    ms = ctypes.c_float(-1.0)
+    if time_kernel:
+        opt.kernel_ms = ctypes.pointer(ms)
+    if on_dev:
+        import torch
+        ach = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
+        asx = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
+    else:
+        ach = np.zeros(nw, np.complex128)
+        asx = np.zeros(nw, np.complex128)
+    _check(lib.gpp_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(ach), _ptr(asx), None))
+    return ach, asx, (float(ms.value) if time_kernel else None)
+
+
+def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0):
+    wx = np.zeros(nfreq)
+    dummy = 1  # non-NULL placeholders: only sizes are read
+    inv = np.zeros(ngpown, np.int32)
+    ind = np.zeros(ncouls + 1, np.int32)
+    prob = GppProblem(nb, nv, ncouls, ngpown, nfreq, wx.ctypes.data, dummy, dummy, dummy, dummy, dummy,
+                      inv.ctypes.data, ind.ctypes.data, 1e-6, 0.25, 2.5e5, 1.0)
+    opt = default_options()
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.config = config
+    return int(lib.gpp_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
+
+
+def gpp_problem(p, **kw):
+    """gpp_kernel on an object with the oracle's GppProblem fields."""
+    return gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, p.aqsm, p.aqsn, p.I_eps, p.wtilde, p.vcoul,
+                      p.inv_igp_index, p.indinv, to1=p.to1, limittwo=p.limittwo, limitone=p.limitone,
+                      occ=p.occ, **kw)
+
+
+# --------------------------------------------------------------------------
+# FF
+# --------------------------------------------------------------------------
+def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index, indinv, *, achs_freq=0,
+              stream=None, workspace=None, config=0, time_kernel=False, want_schDv=True, device=-1):
+    """Run ff_kernel.  Returns (schDt[nb][nfreq], schDv or None, achsDtemp, kernel_ms|None)."""
+    inv = np.ascontiguousarray(inv_igp_index, np.int32)
+    ind = np.ascontiguousarray(indinv, np.int32)
+    on_dev = _is_torch(aqsn)
+    prob = FfProblem(int(nb), int(ncouls), int(ngpown), int(nfreq), _ptr(aqsm), _ptr(aqsn), _ptr(I_eps),
+                     _ptr(vcoul), inv.ctypes.data, ind.ctypes.data, int(achs_freq))
+    opt = default_options()
+    opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
+    opt.device = int(device)
+    opt.stream = _stream_handle(stream)
+    if workspace is not None:
+        opt.workspace = _ptr(workspace)
+        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.config = int(config)
+    ms = ctypes.c_float(-1.0)
+    if time_kernel:
+        opt.kernel_ms = ctypes.pointer(ms)
+    if on_dev:
+        import torch
+        mk = lambda *s: torch.empty(*s, dtype=torch.complex128, device=aqsn.device)  # noqa: E731
+    else:
+        mk = lambda *s: np.zeros(s, np.complex128)  # noqa: E731
+    schDt = mk(nb, nfreq)
+    schDv = mk(nb, nfreq) if want_schDv else None
+    achs = mk(1)
+    _check(lib.ff_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(schDt), _ptr(schDv), _ptr(achs)))
+    return schDt, schDv, achs, (float(ms.value) if time_kernel else None)
+
+
+def ff_problem(p, **kw):
+    return ff_kernel(p.nb, p.ncouls, p.ngpown, p.nfreq, p.aqsm, p.aqsn, p.I_eps, p.vcoul, p.inv_igp_index,
+                     p.indinv, **kw)
+
+
+# --------------------------------------------------------------------------
+# device fill / complex helpers / diagnostics
+# --------------------------------------------------------------------------
+def fill_complex(dst, seed, stream_id, re, im, stream=None):
+    """Counter-based fill of a complex128 or complex64 device tensor."""
+    import torch
+    as64 = 1 if dst.dtype == torch.complex64 else 0
+    _check(lib.fk_fill_complex(_ptr(dst), dst.numel(), seed, stream_id, re[0], re[1], im[0], im[1], as64,
+                               _stream_handle(stream)))
+
+
+def fill_real(dst, seed, stream_id, lo, hi, stream=None):
+    _check(lib.fk_fill_real(_ptr(dst), dst.numel(), seed, stream_id, lo, hi, _stream_handle(stream)))
+
+
+def complex_apply(op, a=None, b=None, s=None, out=None, on_device=False):
+    n = len(next(x for x in (a, b, s, out) if x is not None))
+    a = None if a is None else np.ascontiguousarray(a, np.complex128)
+    b = None if b is None else np.ascontiguousarray(b, np.complex128)
+    s = None if s is None else np.ascontiguousarray(s, np.float64)
+    out = np.zeros(n, np.complex128) if out is None else np.array(out, np.complex128)
+    _check(lib.fk_complex_apply(COP[op], n, _ptr(a), _ptr(b), _ptr(s), _ptr(out), int(on_device)))
+    return out
+
+
+def mfma_probe(kind):
+    """kind 0: FP64 16x16x4 MFMA, kind 1: FP32 16x16x4 MFMA; returns [64 lanes][4 registers]."""
+    out = np.zeros(256, np.float64)
+    _check(lib.fk_mfma_probe(int(kind), out.ctypes.data))
+    return out.reshape(64, 4)

@@ -1,0 +1,84 @@
+// fk_internal.h -- shared host-side plumbing of the C ABI: error reporting,
+// HIP checks, device scratch and the column-map resolution.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/fortkernels.h"
+
+namespace fk {
+
+// Message of the last failing call on this thread.
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+#define FK_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t fk_e_ = (expr);                                                \
+        if (fk_e_ != hipSuccess) {                                                \
+            ::fk::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(fk_e_), \
+                            __FILE__, __LINE__);                                  \
+            return FK_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define FK_TRY(expr)                    \
+    do {                                \
+        int fk_rc_ = (expr);            \
+        if (fk_rc_ != FK_OK) return fk_rc_; \
+    } while (0)
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline int div_up(int x, int a) { return (x + a - 1) / a; }
+
+// Column map of BGW-Kernels/gppKer.cpp:322-328: igp = indinv[inv_igp_index[my_igp]],
+// indigp == ncouls -> ncouls-1, igp > ncouls or igp < 0 -> -1 (column skipped).
+// Returns FK_ERR_INDEX where the reference would read out of bounds
+// (indigp outside [0, ncouls], or igp == ncouls).
+int resolve_columns(int ncouls, int ngpown, const int32_t *inv_igp_index, const int32_t *indinv,
+                    std::vector<int32_t> &igp_out);
+
+// Device scratch: a caller-provided workspace or a stream-ordered allocation
+// released by the destructor (stream-ordered as well).
+class Scratch {
+public:
+    Scratch() = default;
+    ~Scratch();
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    // Carve `bytes` (256-aligned offsets).  Call plan() for every piece, then
+    // acquire(), then ptr() in the same order.
+    size_t plan(size_t bytes) { size_t off = total_; total_ = align_up(total_ + bytes, 256); return off; }
+    size_t total() const { return total_; }
+    int acquire(void *user, size_t user_bytes, hipStream_t stream);
+    void *at(size_t offset) const { return static_cast<char *>(base_) + offset; }
+
+private:
+    size_t total_ = 0;
+    void *base_ = nullptr;
+    bool owned_ = false;
+    hipStream_t stream_ = nullptr;
+};
+
+// Timing of one kernel with HIP events on its stream (only when requested).
+class KernelTimer {
+public:
+    KernelTimer(float *out, hipStream_t s);
+    ~KernelTimer();
+    int start();
+    int stop();
+    int finish();   // synchronise and write the elapsed ms
+
+private:
+    float *out_;
+    hipStream_t s_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+}  // namespace fk

@@ -1,0 +1,222 @@
+/*
+ * fortkernels.h -- C ABI of the MI355X (gfx950) GPP / FF self-energy kernels.
+ *
+ * A drop-in for the BerkeleyGW mini-app loops of rahulgayatri23/FortKernels-:
+ *   gpp_kernel()  replaces the GPP band loop of BGW-Kernels/gppKer.cpp:303-439
+ *                 (the canonical CPU semantics), i.e. the work of
+ *                 flagOCC_solver / ssxt_scht_solver (:19-65, :130-159) and of
+ *                 the GPU-style achtemp_kernel / tillNvband_kernel
+ *                 (gppKer_gpuComplex.cpp:127-206) and the OpenMP-target
+ *                 kernels (BGW-Kernels/gppKer_gpuComplexGCCTarget.cpp:355-410).
+ *   ff_kernel()   replaces the FF frequency contraction, zgemm + conj
+ *                 contraction of ffkernel.deslippe.best.f90.back1:402-418,
+ *                 and the static-CH sum achsDtemp_Kernel
+ *                 (FFKer_cori/ffKernel.cpp:107-131; Fortran
+ *                 ffkernel.deslippe.best.f90:251-267).
+ *
+ * Conventions (the reference's functions are void and accumulate into
+ * caller-zeroed outputs; these entry points instead):
+ *   - return FK_OK (0) or a negative fk_status; fk_last_error() gives a
+ *     message for the calling thread; the library never exits;
+ *   - OVERWRITE their outputs;
+ *   - take plain pointers and sizes; complex values are interleaved
+ *     {re, im} pairs, the memory layout of std::complex<double>, GPUComplex
+ *     and HIP double2;
+ *   - hold no global mutable state: concurrent calls on distinct streams
+ *     are safe; calls sharing a workspace must be serialised by the caller.
+ */
+#ifndef FORTKERNELS_H
+#define FORTKERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_ABI_VERSION 1
+
+typedef struct { double re, im; } fk_cdouble;  /* complex128 */
+typedef struct { float re, im; } fk_cfloat;    /* complex64 (mixed-precision storage) */
+
+typedef enum {
+    FK_OK = 0,
+    FK_ERR_ARG = -1,          /* NULL pointer, bad size or option */
+    FK_ERR_INDEX = -2,        /* column map the reference would read out of bounds */
+    FK_ERR_UNSUPPORTED = -3,  /* option combination not available */
+    FK_ERR_HIP = -4,          /* HIP runtime error (message has the HIP error string) */
+    FK_ERR_NOMEM = -5,        /* device allocation failed */
+    FK_ERR_WORKSPACE = -6     /* caller workspace smaller than *_workspace_size() */
+} fk_status;
+
+typedef enum {
+    FK_MEM_HOST = 0,    /* bulk inputs and outputs are host memory; the call
+                           copies them, runs, copies back and synchronises */
+    FK_MEM_DEVICE = 1   /* bulk inputs and outputs are device memory of the
+                           current HIP device; the call is asynchronous on
+                           `stream` (outputs valid after the stream syncs) */
+} fk_memory;
+
+typedef enum {
+    FK_PREC_FP64 = 0,   /* complex128 inputs, FP64 arithmetic throughout */
+    FK_PREC_MIXED = 1   /* complex64 inputs (fk_cfloat): band contraction in
+                           FP32 with FP32 accumulation, pole factors and
+                           every cross-band / cross-tile sum in FP64 */
+} fk_precision;
+
+typedef struct {
+    int32_t memory;          /* fk_memory */
+    int32_t precision;       /* fk_precision */
+    int32_t mask;            /* 1: canonical near-pole mask on the achtemp sum
+                                (BGW-Kernels/gppKer.cpp:406-407);
+                                0: unmasked, the GPU-variant semantics
+                                (gppKer_gpuComplex.cpp:171-178) */
+    int32_t device;          /* HIP device ordinal for FK_MEM_HOST (-1 = current) */
+    void *stream;            /* hipStream_t; NULL = the null stream */
+    void *workspace;         /* device scratch; NULL = allocated per call
+                                (stream-ordered) */
+    size_t workspace_bytes;
+    int32_t config;          /* kernel tile configuration; 0 = choose by shape */
+    float *kernel_ms;        /* if non-NULL: the call times its dominant kernel
+                                with HIP events, synchronises, and stores the
+                                elapsed milliseconds here */
+} fk_options;
+
+/* Zero-initialised options with the canonical defaults (mask on, FP64,
+ * host memory, current device, null stream). */
+fk_options fk_default_options(void);
+
+/* ------------------------------------------------------------------------
+ * GPP: achtemp / asxtemp of the band loop.
+ *   achtemp[iw] = sum_{n1<nb} sum_{my_igp} 0.5 vcoul[igp] sum_ig
+ *                 mask * conj(aqsm[n1][igp]) aqsn[n1][ig] delw I_eps[my_igp][ig]
+ *   asxtemp[iw] = sum_{n1<nv} sum_{my_igp} occ vcoul[igp] sum_ig
+ *                 aqsn[n1][ig] conj(aqsm[n1][igp]) ssx(three-way branch)
+ * with wdiff = wx[iw] - wtilde, delw = wtilde conj(wdiff)/|wdiff|^2,
+ * mask = |wdiff|^2 > limittwo && |delw|^2 < limitone, and
+ * igp = indinv[inv_igp_index[my_igp]] (indigp == ncouls -> igp = ncouls-1;
+ * igp > ncouls or < 0 -> column skipped), BGW-Kernels/gppKer.cpp:322-328.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t number_bands;        /* nb */
+    int32_t nvband;              /* nv <= nb occupied bands */
+    int32_t ncouls;              /* C, G-vectors */
+    int32_t ngpown;              /* G, columns owned (ncouls / nodes_per_group) */
+    int32_t nfreq;               /* nw frequencies, 1..16 (reference: 3) */
+    const double *wx;            /* [nfreq] HOST: wx_array (already clamped) */
+    const void *aqsmtemp;        /* [nb][ncouls]  fk_cdouble (FP64) | fk_cfloat (MIXED) */
+    const void *aqsntemp;        /* [nb][ncouls]  same element type */
+    const void *I_eps_array;     /* [ngpown][ncouls] same element type */
+    const void *wtilde_array;    /* [ngpown][ncouls] same element type */
+    const double *vcoul;         /* [ncouls] */
+    const int32_t *inv_igp_index;/* [ngpown]   HOST */
+    const int32_t *indinv;       /* [ncouls+1] HOST */
+    double to1;                  /* 1e-6 */
+    double limittwo;             /* 0.25 */
+    double limitone;             /* 1/(4 to1) */
+    double occ;                  /* 1.0 */
+} fk_gpp_problem;
+
+/* Replaces BGW-Kernels/gppKer.cpp:303-439 (and gppKer_gpuComplex.cpp:148
+ * achtemp_kernel + :127 tillNvband_kernel).  achtemp, asxtemp: [nfreq].
+ * acht_n1_loc: [number_bands] or NULL -- the per-band achtemp at iw = 2
+ * (:424; needs nfreq >= 3). */
+int gpp_kernel(const fk_gpp_problem *problem, const fk_options *options,
+               fk_cdouble *achtemp, fk_cdouble *asxtemp, fk_cdouble *acht_n1_loc);
+
+/* Device scratch bytes gpp_kernel needs for this problem and options. */
+size_t gpp_kernel_workspace_size(const fk_gpp_problem *problem, const fk_options *options);
+
+/* ------------------------------------------------------------------------
+ * FF contraction:
+ *   schDt[n1][f] = sum_my_igp conj(aqsm[n1][igp]) sum_ig I_eps[f][my_igp][ig] aqsn[n1][ig]
+ *   schDv[n1][f] = the same with each column weighted by vcoul[igp]
+ *   achsDtemp    = -0.5 * sum_n1 schDv[n1][achs_freq]
+ * (ffkernel.deslippe.best.f90.back1:402-418; :251-267).  Skipped columns
+ * (same rule as GPP) contribute zero.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t number_bands;        /* nb */
+    int32_t ncouls;              /* C */
+    int32_t ngpown;              /* G */
+    int32_t nfreq;               /* nFreq */
+    const void *aqsmtemp;        /* [nb][ncouls] fk_cdouble */
+    const void *aqsntemp;        /* [nb][ncouls] fk_cdouble */
+    const void *I_eps_array;     /* [nfreq][ngpown][ncouls] fk_cdouble (I_epsR or I_epsA) */
+    const double *vcoul;         /* [ncouls] */
+    const int32_t *inv_igp_index;/* [ngpown]   HOST */
+    const int32_t *indinv;       /* [ncouls+1] HOST */
+    int32_t achs_freq;           /* frequency of the static-CH sum: 0 as the
+                                    Fortran original, 1 as the C++ port (:122) */
+} fk_ff_problem;
+
+/* Replaces the zgemm + contraction of ffkernel.deslippe.best.f90.back1:402-418
+ * and achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:107).  schDt: [nb][nfreq];
+ * schDv: [nb][nfreq] or NULL; achsDtemp: one value or NULL. */
+int ff_kernel(const fk_ff_problem *problem, const fk_options *options,
+              fk_cdouble *schDt, fk_cdouble *schDv, fk_cdouble *achsDtemp);
+
+size_t ff_kernel_workspace_size(const fk_ff_problem *problem, const fk_options *options);
+
+/* ------------------------------------------------------------------------
+ * Utilities
+ * ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *fk_last_error(void);
+
+/* Library ABI version (FK_ABI_VERSION) and the gfx target it was built for. */
+int fk_abi_version(void);
+const char *fk_build_target(void);
+
+/* Deterministic synthetic inputs on the device: element j of a complex
+ * array takes re = U(re_lo,re_hi) at counter 2j and im = U(im_lo,im_hi) at
+ * 2j+1 of a SplitMix64 hash keyed by (seed, stream_id); real arrays use
+ * counter j.  Bit-identical to the host generator the tests use.
+ * as_complex64 != 0 stores fk_cfloat (rounded to nearest) instead of
+ * fk_cdouble.  Asynchronous on `stream`.  No counterpart in the reference,
+ * which fills constants (BGW-Kernels/gppKer.cpp:275-291). */
+int fk_fill_complex(void *dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                    double re_lo, double re_hi, double im_lo, double im_hi,
+                    int32_t as_complex64, void *stream);
+int fk_fill_real(double *dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                 double lo, double hi, void *stream);
+
+/* The HIP complex helpers that replace GPUComplex / CustomComplex
+ * (BGW-Kernels/GPUComplex.h:138-247, FFKer_cori/Complex.h:208-311,
+ * FFKer_cori/CustomComplex.h:99-214), applied elementwise on host memory
+ * either by the host instantiation (on_device = 0) or by a device kernel
+ * (on_device = 1).  op: see fk_complex_op.  out[i] = op(a[i], b[i], s[i]);
+ * FMA/FMS read out[i] as the accumulator.  Unused operands may be NULL. */
+typedef enum {
+    FK_COP_MUL = 0,        /* a*b                  GPUComplex_product          */
+    FK_COP_CONJ = 1,       /* conj(a)              GPUComplex_conj             */
+    FK_COP_ABS2 = 2,       /* |a|^2 (re), 0 (im)   GPUComplex_target abs       */
+    FK_COP_ABS = 3,        /* |a| (re), 0 (im)     GPUComplex_abs              */
+    FK_COP_SCALE = 4,      /* a*s                  GPUComplex_mult(a, b)       */
+    FK_COP_FMA = 5,        /* out += a*b (out is read) GPUComplex_fma(out,a,b) */
+    FK_COP_FMS = 6,        /* out -= a*b (out is read) GPUComplex_fms(out,a,b) */
+    FK_COP_REAL_MINUS = 7, /* s - a                doubleMinusGPUComplex       */
+    FK_COP_REAL_PLUS = 8,  /* s + a                doublePlusGPUComplex        */
+    FK_COP_DIV = 9,        /* a/b                  GPUComplex_divide(a, b)     */
+    FK_COP_DIV_REAL = 10,  /* a/s                  GPUComplex_divide(a, double)*/
+    FK_COP_SQUARE = 11,    /* a*a                  GPUComplex_square           */
+    FK_COP_ADD = 12,       /* a+b                                              */
+    FK_COP_SUB = 13        /* a-b                  GPUComplex_minus            */
+} fk_complex_op;
+int fk_complex_apply(int32_t op, int64_t n, const fk_cdouble *a, const fk_cdouble *b,
+                     const double *s, fk_cdouble *out, int32_t on_device);
+
+/* Diagnostic with no counterpart in the reference: the raw accumulator of
+ * one 16x16x4 MFMA (kind 0: FP64, kind 1: FP32) on fixed operands
+ * A[i][k] = 1 + i + 16k, B[k][j] = 1 + 3j - k, where lane l supplies
+ * A[l%16][l/16] and B[l/16][l%16]; out[4*lane + r] = accumulator register r
+ * of lane `lane` (256 doubles, host memory).  Tests compare it with the
+ * result layout the GEMM kernels assume. */
+int fk_mfma_probe(int32_t kind, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FORTKERNELS_H */

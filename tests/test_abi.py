@@ -1,0 +1,100 @@
+"""C-ABI library: it loads, exports every symbol the header declares, and
+rejects bad problems on the host side (no GPU needed for any of these)."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+
+def _header_symbols(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, re.M)
+    return sorted(set(n for n in names if n not in ("if", "sizeof")))
+
+
+def test_library_exports_every_declared_symbol(fk):
+    declared = _header_symbols(fk.INCLUDE)
+    assert "gpp_kernel" in declared and "ff_kernel" in declared
+    for name in declared:
+        assert hasattr(fk.lib, name), f"missing export {name}"
+    assert sorted(fk.EXPORTS) == declared
+
+
+def test_abi_version_and_target(fk):
+    assert fk.lib.fk_abi_version() == 1
+    assert fk.lib.fk_build_target() == b"gfx950"
+
+
+def test_default_options(fk):
+    o = fk.default_options()
+    assert (o.memory, o.precision, o.mask, o.device, o.config) == (fk.FK_MEM_HOST, fk.FK_PREC_FP64, 1, -1, 0)
+    assert not o.stream and not o.workspace and not o.kernel_ms
+
+
+def _small(fko):
+    return fko.gpp_random(4, 2, 64, 5, seed=1)
+
+
+def test_gpp_rejects_bad_sizes_before_touching_a_device(fk, fko):
+    p = _small(fko)
+    with pytest.raises(fk.FkError) as e:
+        fk.gpp_kernel(p.nb, p.nb + 1, p.ncouls, p.ngpown, p.wx, p.aqsm, p.aqsn, p.I_eps, p.wtilde, p.vcoul,
+                      p.inv_igp_index, p.indinv)
+    assert e.value.code == fk.FK_ERR_ARG
+    with pytest.raises(fk.FkError) as e:
+        fk.gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, np.zeros(17), p.aqsm, p.aqsn, p.I_eps, p.wtilde, p.vcoul,
+                      p.inv_igp_index, p.indinv)
+    assert e.value.code == fk.FK_ERR_ARG
+
+
+@pytest.mark.parametrize("bad", ["indigp_high", "indigp_neg", "igp_eq_ncouls"])
+def test_gpp_rejects_out_of_range_column_maps(fk, fko, bad):
+    """Index maps the reference would read out of bounds (BGW-Kernels/gppKer.cpp:322-328)."""
+    p = _small(fko)
+    inv, ind = p.inv_igp_index.copy(), p.indinv.copy()
+    if bad == "indigp_high":
+        inv[1] = p.ncouls + 1
+    elif bad == "indigp_neg":
+        inv[0] = -1
+    else:
+        ind[inv[2]] = p.ncouls
+    with pytest.raises(fk.FkError) as e:
+        fk.gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, p.aqsm, p.aqsn, p.I_eps, p.wtilde, p.vcoul, inv, ind)
+    assert e.value.code == fk.FK_ERR_INDEX
+    assert "ncouls" in fk.last_error()
+
+
+def test_gpp_workspace_size_grows_with_problem(fk):
+    small = fk.gpp_workspace_size(128, 2, 8192, 409, 3)
+    big = fk.gpp_workspace_size(512, 2, 32768, 1638, 3)
+    assert 0 < small < big
+    assert big < 64 << 20   # coefficient matrix + per-tile partials only
+
+
+def test_complex_helpers_host(fk):
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=64) + 1j * rng.normal(size=64)
+    b = rng.normal(size=64) + 1j * rng.normal(size=64)
+    s = rng.uniform(0.5, 2.0, 64)
+    acc = rng.normal(size=64) + 1j * rng.normal(size=64)
+    ap = lambda op, **kw: fk.complex_apply(op, **kw)  # noqa: E731
+    assert np.allclose(ap("mul", a=a, b=b), a * b, rtol=1e-15)
+    assert np.array_equal(ap("conj", a=a), np.conj(a))
+    assert np.allclose(ap("abs2", a=a).real, np.abs(a) ** 2, rtol=1e-15)
+    assert np.allclose(ap("abs", a=a).real, np.abs(a), rtol=1e-15)
+    assert np.allclose(ap("scale", a=a, s=s), a * s, rtol=1e-15)
+    assert np.allclose(ap("fma", a=a, b=b, out=acc), acc + a * b, rtol=1e-14)
+    assert np.allclose(ap("fms", a=a, b=b, out=acc), acc - a * b, rtol=1e-14)
+    assert np.allclose(ap("real_minus", a=a, s=s), s - a, rtol=1e-15)
+    assert np.allclose(ap("real_plus", a=a, s=s), s + a, rtol=1e-15)
+    assert np.allclose(ap("div", a=a, b=b), a / b, rtol=1e-14)
+    assert np.allclose(ap("div_real", a=a, s=s), a / s, rtol=1e-15)
+    assert np.allclose(ap("square", a=a), a * a, rtol=1e-15)
+    assert np.allclose(ap("add", a=a, b=b), a + b, rtol=1e-15)
+    assert np.allclose(ap("sub", a=a, b=b), a - b, rtol=1e-15)
+    # identities: conj(ab) = conj(a) conj(b); a / a = 1; |a|^2 = a conj(a)
+    assert np.allclose(ap("conj", a=ap("mul", a=a, b=b)), ap("mul", a=np.conj(a), b=np.conj(b)), rtol=1e-15)
+    assert np.allclose(ap("div", a=a, b=a), np.ones(64), rtol=1e-15, atol=1e-15)
+    assert np.allclose(ap("mul", a=a, b=np.conj(a)).real, ap("abs2", a=a).real, rtol=1e-15)
