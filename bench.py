@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the GPP band sum is benchmarked per device (a multi-GPU run "
@@ -220,6 +222,14 @@ def main():
                          "achieved_tflops": ach_m, "peak_fp32_tflops": PEAK_FP32,
                          "workload": "same shape, complex64 storage, wtilde re ~ U(3,5)"}
         log(f"mixed: {m['ms_per_step']:.3f} ms/step ({line['mixed']['speedup_vs_fp64']:.2f}x vs fp64)")
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 x {args.calibrate_copy} MiB")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_bands, args.cpu_threads)
     print(json.dumps(line), flush=True)

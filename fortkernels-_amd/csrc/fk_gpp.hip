@@ -32,7 +32,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxFreq = 16;
-constexpr int kChunk = 4;      // frequencies per epilogue pass
 constexpr int kXcds = 8;       // MI355X: consecutive workgroups go to different XCDs
 
 struct PoleConst {
@@ -93,35 +92,42 @@ __device__ __forceinline__ void tile_coords(int b, int tiles_m, int total, int &
     tn = t / tiles_m;
 }
 
-// Pole factor D (achtemp) and SX factor ssx (asxtemp) of one (my_igp, ig, iw)
-// point, in the reference's FP64 evaluation order (ssxt_scht_solver,
-// BGW-Kernels/gppKer.cpp:19-65; unoccupied loop :396-406).  Contraction is
-// off so |wdiff|^2, delw and |delw|^2 round exactly as the reference's and
-// the mask/branch decisions are identical to it.
-__device__ __forceinline__ void pole_point(double wx, double wx2, double rden1, double2 wt, double2 ie, double2 wt2,
-                                           double2 Om, double limittwo, double limitone, double to1, bool mask_on,
-                                           bool want_ssx, double2 &D, double2 &S)
+// 1/x from v_rcp_f64 refined by two Newton steps (within an ulp or two of
+// the IEEE quotient).
+__device__ __forceinline__ double rcp_fast(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// delw = wtilde * conj(wdiff) * rden and |delw|^2 in the reference's FP64
+// evaluation order (BGW-Kernels/gppKer.cpp:30-36, :396-402).  Contraction is
+// off so, given the same rden, they round exactly as the reference's.
+__device__ __forceinline__ void pole_delw(double2 wt, double wdr, double rden, double2 &delw, double &delwr)
 {
 #pragma clang fp contract(off)
-    double wdr = wx - wt.x, wdi = -wt.y;                       // wdiff = wxt - wtilde
-    double wdiffr = wdr * wdr + wdi * wdi;                     // real(wdiff*conj(wdiff))
-    double rden = 1.0 / wdiffr;
-    double2 delw = make_double2((wt.x * wdr - wt.y * (-wdi)) * rden,   // wtilde*conj(cden)*rden
-                                (wt.x * (-wdi) + wt.y * wdr) * rden);
-    double delwr = delw.x * delw.x + delw.y * delw.y;
-    bool m = (wdiffr > limittwo) && (delwr < limitone);
-    D = (m || !mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
-    S = make_double2(0.0, 0.0);
-    if (want_ssx) {
-        if (m) {                                               // :41-44
-            S = make_double2(Om.x * wx2 * rden1, Om.y * wx2 * rden1);
-        } else if (delwr > to1) {                              // :49-52
-            double2 cden = cmul(make_double2(4.0 * wt2.x, 4.0 * wt2.y), make_double2(delw.x + 0.5, delw.y));
-            double rd = 1.0 / (cden.x * cden.x + cden.y * cden.y);
-            double2 t = cmul(make_double2(-Om.x, -Om.y), cconj(cden));
-            S = cmul(make_double2(t.x * rd, t.y * rd), delw);
-        }
-    }
+    const double wdi = -wt.y;
+    delw = make_double2((wt.x * wdr - wt.y * (-wdi)) * rden, (wt.x * (-wdi) + wt.y * wdr) * rden);
+    delwr = delw.x * delw.x + delw.y * delw.y;
+}
+
+__device__ __forceinline__ double pole_wdiffr(double wdr, double wdi)
+{
+#pragma clang fp contract(off)
+    return wdr * wdr + wdi * wdi;
+}
+
+// ssx of the near-pole branch (:46-53): -Omega2 conj(cden)/|cden|^2 delw,
+// cden = 4 wtilde^2 (delw + 0.5).
+__device__ __forceinline__ double2 ssx_branch2(double2 wt2, double2 Om, double2 delw)
+{
+    double2 cden = cmul(make_double2(4.0 * wt2.x, 4.0 * wt2.y), make_double2(delw.x + 0.5, delw.y));
+    double rd = 1.0 / (cden.x * cden.x + cden.y * cden.y);
+    double2 t = cmul(make_double2(-Om.x, -Om.y), cconj(cden));
+    return cmul(make_double2(t.x * rd, t.y * rd), delw);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -172,29 +178,34 @@ __global__ void gpp_coef_kernel(CoefArgs<E> a)
 }
 
 // ---------------------------------------------------------------------------
-// Fused GEMM tile + epilogue.  256 threads = 4 waves in a 2x2 arrangement;
-// workgroup tile BM (my_igp) x BN (ig); wave tile BM/2 x BN/2 made of 16x16
-// MFMA blocks.  MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; accumulator register r of lane l is C[acc_row(l,r)][l%16].
+// Fused GEMM tile + epilogue.  WGM x WGN waves per workgroup; workgroup tile
+// BM (my_igp) x BN (ig); wave tile BM/WGM x BN/WGN made of 16x16 MFMA blocks;
+// MINWG workgroups per CU; NWC frequencies per epilogue pass.
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register r of lane l is C[acc_row(l,r)][l%16].
 // Complex products use four real MFMAs (re += ar br - ai bi, im += ar bi + ai br).
 // ---------------------------------------------------------------------------
-template <class E, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void gpp_fused_kernel(GppArgs<E> a)
+template <class E, int BM, int BN, int BK, int WGM, int WGN, int MINWG, int NWC>
+__global__ __launch_bounds__(64 * WGM * WGN, MINWG) void gpp_fused_kernel(GppArgs<E> a)
 {
     using A4 = typename Acc<E>::type;
-    constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
-    constexpr int LA = BK * BM / 256, LB = BK * BN / 256;
-    static_assert(BK % 4 == 0 && LA * 256 == BK * BM && LB * 256 == BK * BN, "tile shape");
+    constexpr int NT = 64 * WGM * WGN, NWAVES = WGM * WGN;
+    constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
+    constexpr int LA = BK * BM / NT, LB = BK * BN / NT;
+    constexpr int KS = BK / 4;
+    static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "tile shape");
+    static_assert(LA * NT == BK * BM && LB * NT == BK * BN, "tile loads");
 
     __shared__ E As[2][BK * BM];
     __shared__ E Bs[2][BK * BN];
-    __shared__ double red[4][4 * kChunk];
+    __shared__ double red[NWAVES][4 * NWC];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WGM, wn = wave / WGM;
     int tm, tn;
     tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
+    const int kmax = a.nb - 1, nmax = a.C - 1;
 
     A4 acc_re[MB][NB], acc_im[MB][NB];
 #pragma unroll
@@ -205,24 +216,29 @@ __global__ __launch_bounds__(256) void gpp_fused_kernel(GppArgs<E> a)
             acc_im[i][j] = A4{0, 0, 0, 0};
         }
 
+    // Global -> register staging.  c is zero-padded to Kpad x ldc, so its
+    // loads need no guard; aqsn loads read a clamped (always valid) address
+    // and select zero outside [0,nb) x [0,C), which keeps the loop free of
+    // divergent branches.
     E ra[LA], rb[LB];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
-            int idx = tid + 256 * i, kk = idx / BM, mm = idx % BM;
+            int idx = tid + NT * i, kk = idx / BM, mm = idx % BM;
             ra[i] = a.c[(size_t)(k0 + kk) * a.ldc + m0 + mm];
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            int idx = tid + 256 * i, k = k0 + idx / BN, n = n0 + idx % BN;
-            rb[i] = (k < a.nb && n < a.C) ? a.aqsn[(size_t)k * a.C + n] : zero_elem<E>();
+            int idx = tid + NT * i, k = k0 + idx / BN, n = n0 + idx % BN;
+            E v = a.aqsn[(size_t)min(k, kmax) * a.C + min(n, nmax)];
+            rb[i] = (k <= kmax && n <= nmax) ? v : zero_elem<E>();
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) As[buf][tid + 256 * i] = ra[i];
+        for (int i = 0; i < LA; ++i) As[buf][tid + NT * i] = ra[i];
 #pragma unroll
-        for (int i = 0; i < LB; ++i) Bs[buf][tid + 256 * i] = rb[i];
+        for (int i = 0; i < LB; ++i) Bs[buf][tid + NT * i] = rb[i];
     };
 
     const int nk = a.Kpad / BK;
@@ -233,7 +249,7 @@ __global__ __launch_bounds__(256) void gpp_fused_kernel(GppArgs<E> a)
         const int buf = kt & 1;
         if (kt + 1 < nk) gload((kt + 1) * BK);
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
+        for (int ks = 0; ks < KS; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
             E af[MB], bf[NB];
 #pragma unroll
@@ -259,49 +275,116 @@ __global__ __launch_bounds__(256) void gpp_fused_kernel(GppArgs<E> a)
         __syncthreads();
     }
 
-    // ---- epilogue: pole / SX factors and the reduction over the tile ----
+    // ---- epilogue ----------------------------------------------------------
+    // Per (my_igp, ig) of the tile: B from the accumulators, Bocc (the same
+    // contraction over the occupied bands n1 < nv) from a few MFMAs on the
+    // occupied bands staged in LDS, then per frequency the pole factor D and
+    // the SX factor ssx, accumulated per lane and reduced over the tile.
     const bool want_ssx = a.nv > 0;
-    const size_t tile = (size_t)tm + (size_t)tn * a.tiles_m;
-    for (int w0 = 0; w0 < a.nw; w0 += kChunk) {
-        double2 sach[kChunk], sasx[kChunk];
+    const int nvpad = (a.nv + 3) & ~3;
+    const bool occ_lds = nvpad <= 2 * BK;      // both LDS stages hold the occupied bands
+    E *Aocc = &As[0][0], *Bocc_s = &Bs[0][0];   // [2*BK][BM] and [2*BK][BN], contiguous
+    if (want_ssx && occ_lds) {
+        for (int idx = tid; idx < nvpad * BM; idx += NT) {
+            int k = idx / BM, mm = idx % BM;
+            E v = a.c[(size_t)k * a.ldc + m0 + mm];
+            Aocc[idx] = k < a.nv ? v : zero_elem<E>();
+        }
+        for (int idx = tid; idx < nvpad * BN; idx += NT) {
+            int k = idx / BN, n = n0 + idx % BN;
+            E v = a.aqsn[(size_t)min(k, a.nv - 1) * a.C + min(n, nmax)];
+            Bocc_s[idx] = (k < a.nv && n <= nmax) ? v : zero_elem<E>();
+        }
+    }
+    __syncthreads();
+
+    // rows of this lane that are real, kept columns
+    unsigned rowok = 0;
 #pragma unroll
-        for (int q = 0; q < kChunk; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = m0 + wm * WM + i * 16 + acc_row<E>(lane, r);
+            const bool ok = row < a.G && a.igp[min(row, a.G - 1)] >= 0;
+            rowok |= (ok ? 1u : 0u) << (i * 4 + r);
+        }
+
+    const size_t tile = (size_t)tm + (size_t)tn * a.tiles_m;
+    const double tol1 = 1e-13 * a.limitone, tol2 = 1e-13 * a.to1;
+    for (int w0 = 0; w0 < a.nw; w0 += NWC) {
+        double2 sach[NWC], sasx[NWC];
+#pragma unroll
+        for (int q = 0; q < NWC; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * WM + i * 16 + acc_row<E>(lane, r);
-                if (row >= a.G || a.igp[row] < 0) continue;
+            for (int j = 0; j < NB; ++j) {
+                A4 oc_re = A4{0, 0, 0, 0}, oc_im = A4{0, 0, 0, 0};
+                if (want_ssx && occ_lds) {
+                    for (int ks = 0; ks < nvpad / 4; ++ks) {
+                        const int kr = ks * 4 + (lane >> 4);
+                        E af = Aocc[kr * BM + wm * WM + i * 16 + (lane & 15)];
+                        E bf = Bocc_s[kr * BN + wn * WN + j * 16 + (lane & 15)];
+                        oc_re = mfma(af.x, bf.x, oc_re);
+                        oc_im = mfma(af.x, bf.y, oc_im);
+                        oc_re = mfma(-af.y, bf.y, oc_re);
+                        oc_im = mfma(af.y, bf.x, oc_im);
+                    }
+                }
+                const int col = n0 + wn * WN + j * 16 + (lane & 15);
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    const int col = n0 + wn * WN + j * 16 + (lane & 15);
-                    if (col >= a.C) continue;
-                    const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
-                    const size_t e = (size_t)row * a.C + col;
+                for (int r = 0; r < 4; ++r) {
+                    const int row = m0 + wm * WM + i * 16 + acc_row<E>(lane, r);
+                    const bool ok = ((rowok >> (i * 4 + r)) & 1u) && col <= nmax;
+                    const size_t e = (size_t)min(row, a.G - 1) * a.C + min(col, nmax);
                     const double2 wt = widen(a.wtilde[e]);
                     const double2 ie = widen(a.I_eps[e]);
-                    double2 Bocc = make_double2(0.0, 0.0), wt2 = Bocc, Om = Bocc;
-                    if (want_ssx) {
+                    const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
+                    double2 Bo = make_double2((double)oc_re[r], (double)oc_im[r]);
+                    if (want_ssx && !occ_lds) {
+                        Bo = make_double2(0.0, 0.0);
                         for (int n1 = 0; n1 < a.nv; ++n1)
-                            Bocc = cfma(Bocc, widen(a.c[(size_t)n1 * a.ldc + row]), widen(a.aqsn[(size_t)n1 * a.C + col]));
-                        wt2 = cmul(wt, wt);
-                        Om = cmul(wt2, ie);
+                            Bo = cfma(Bo, widen(a.c[(size_t)n1 * a.ldc + min(row, a.ldc - 1)]),
+                                      widen(a.aqsn[(size_t)n1 * a.C + min(col, nmax)]));
                     }
+                    const double2 wt2 = cmul(wt, wt);
+                    const double2 Om = cmul(wt2, ie);
 #pragma unroll
-                    for (int q = 0; q < kChunk; ++q) {
-                        const int iw = w0 + q;
-                        if (iw >= a.nw) break;
-                        double2 D, S;
-                        pole_point(a.pc.wx[iw], a.pc.wx2[iw], a.pc.rden1[iw], wt, ie, wt2, Om, a.limittwo,
-                                   a.limitone, a.to1, a.mask_on != 0, want_ssx, D, S);
-                        sach[q] = cfma(sach[q], B, D);
-                        sasx[q] = cfma(sasx[q], Bocc, S);
+                    for (int q = 0; q < NWC; ++q) {
+                        const int iw = min(w0 + q, a.nw - 1);   // a short last pass repeats a frequency ...
+                        const bool live = ok && (w0 + q < a.nw); // ... and adds nothing for it
+                        const double wdr = a.pc.wx[iw] - wt.x;
+                        const double wdiffr = pole_wdiffr(wdr, -wt.y);
+                        double2 delw;
+                        double delwr;
+                        pole_delw(wt, wdr, rcp_fast(wdiffr), delw, delwr);
+                        // points within 1e-13 of a branch threshold take the IEEE
+                        // quotient, so every branch decision is the reference's
+                        const bool near = fabs(delwr - a.limitone) <= tol1 || fabs(delwr - a.to1) <= tol2;
+                        if (__ballot(near)) {
+                            if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
+                        }
+                        const bool m = (wdiffr > a.limittwo) && (delwr < a.limitone);
+                        const double2 D = (m || !a.mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
+                        const double2 t = cmul(B, D);
+                        if (live) sach[q] = cadd(sach[q], t);
+                        if (want_ssx) {
+                            // mask branch (:41-44): Omega2 * conj(wx^2) / |wx^2|^2
+                            double2 S = m ? make_double2(Om.x * a.pc.wx2[iw] * a.pc.rden1[iw], Om.y * a.pc.wx2[iw] * a.pc.rden1[iw])
+                                          : make_double2(0.0, 0.0);
+                            const bool need2 = !m && delwr > a.to1;
+                            if (__ballot(need2)) {
+                                if (need2) S = ssx_branch2(wt2, Om, delw);
+                            }
+                            const double2 u = cmul(Bo, S);
+                            if (live) sasx[q] = cadd(sasx[q], u);
+                        }
                     }
                 }
             }
-        // wave reduction, then a fixed-order sum over the four waves
+        // wave reduction, then a fixed-order sum over the waves
 #pragma unroll
-        for (int q = 0; q < kChunk; ++q) {
+        for (int q = 0; q < NWC; ++q) {
             double v0 = wave_sum(sach[q].x), v1 = wave_sum(sach[q].y);
             double v2 = wave_sum(sasx[q].x), v3 = wave_sum(sasx[q].y);
             if (lane == 0) {
@@ -312,17 +395,25 @@ __global__ __launch_bounds__(256) void gpp_fused_kernel(GppArgs<E> a)
             }
         }
         __syncthreads();
-        if (tid < 4 * kChunk) {
+        if (tid < 4 * NWC) {
             const int q = tid >> 2, comp = tid & 3;
             const int iw = w0 + q;
             if (iw < a.nw) {
-                double s = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+                double sum = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
                 double *dst = reinterpret_cast<double *>(a.partials + (tile * a.nw + iw) * 2);
-                dst[comp] = s;   // comp 0,1: achtemp re,im; 2,3: asxtemp re,im
+                dst[comp] = sum;   // comp 0,1: achtemp re,im; 2,3: asxtemp re,im
             }
         }
         __syncthreads();
     }
+}
+
+__global__ void zero_outputs_kernel(double2 *a, double2 *b, int n)
+{
+    int i = threadIdx.x;
+    if (i < n) { a[i] = make_double2(0.0, 0.0); b[i] = make_double2(0.0, 0.0); }
 }
 
 // One workgroup per output value: sums the per-tile partials in a fixed order.
@@ -358,15 +449,44 @@ __global__ __launch_bounds__(256) void gpp_reduce_kernel(const double2 *partials
 // host side
 // ---------------------------------------------------------------------------
 struct TileCfg { int bm, bn, bk; };
-constexpr TileCfg kCfgs[] = {{64, 128, 16}, {32, 64, 16}};
+// config ids (fk_options.config = id + 1):
+//   0: 64x128, BK 16, 2x2 waves, 1 workgroup/CU
+//   1: 32x64,  BK 16, 2x2 waves, 1 workgroup/CU   (small problems)
+//   2: 64x128, BK 8,  2x2 waves, 2 workgroups/CU
+//   3: 64x128, BK 16, 2x4 waves, 1 workgroup/CU
+constexpr TileCfg kCfgs[] = {{64, 128, 16}, {32, 64, 16}, {64, 128, 8}, {64, 128, 16}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+template <class E, int NWC>
+void launch_fused_nw(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
+{
+    switch (cfg) {
+    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 16, 2, 2, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((gpp_fused_kernel<E, 32, 64, 16, 2, 2, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 8, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 16, 2, 4, 1, NWC>), dim3(tiles), dim3(512), 0, s, a); break;
+    }
+}
+
+// Frequencies per epilogue pass: all of them up to 4, else passes of 4.
+template <class E>
+void launch_fused(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
+{
+    switch (a.nw) {
+    case 1: launch_fused_nw<E, 1>(cfg, tiles, s, a); break;
+    case 2: launch_fused_nw<E, 2>(cfg, tiles, s, a); break;
+    case 3: launch_fused_nw<E, 3>(cfg, tiles, s, a); break;
+    default: launch_fused_nw<E, 4>(cfg, tiles, s, a); break;
+    }
+}
 
 int choose_cfg(const fk_gpp_problem *p, const fk_options *o)
 {
     if (o->config > 0) return o->config - 1;
-    // the big tile unless it leaves the 256 CUs with under two tiles each
-    long tiles = (long)div_up(p->ngpown, kCfgs[0].bm) * div_up(p->ncouls, kCfgs[0].bn);
-    return tiles >= 512 ? 0 : 1;
+    // 64x128 tiles, two workgroups per CU, unless that leaves the 256 CUs
+    // with under two tiles each; then the 32x64 tile
+    long tiles = (long)div_up(p->ngpown, kCfgs[2].bm) * div_up(p->ncouls, kCfgs[2].bn);
+    return tiles >= 512 ? 2 : 1;
 }
 
 struct Layout {
@@ -417,6 +537,11 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
                const double *vcoul, double2 *d_ach, double2 *d_asx, hipStream_t s)
 {
     const int C = p->ncouls, G = p->ngpown;
+    if (p->number_bands == 0) {   // no band: both sums are empty
+        hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, d_ach, d_asx, p->nfreq);
+        FK_HIP(hipGetLastError());
+        return FK_OK;
+    }
     E *c = static_cast<E *>(ws.at(L.off_c));
     int *igp = static_cast<int *>(ws.at(L.off_igp));
     double2 *part = static_cast<double2 *>(ws.at(L.off_part));
@@ -452,10 +577,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     const int tiles = L.tiles_m * L.tiles_n;
     KernelTimer timer(o->kernel_ms, s);
     FK_TRY(timer.start());
-    switch (L.cfg) {
-    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 16>), dim3(tiles), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 32, 64, 16>), dim3(tiles), dim3(256), 0, s, a); break;
-    }
+    launch_fused<E>(L.cfg, tiles, s, a);
     FK_HIP(hipGetLastError());
     FK_TRY(timer.stop());
     hipLaunchKernelGGL(gpp_reduce_kernel, dim3(2 * p->nfreq), dim3(256), 0, s, part, tiles, p->nfreq,
