@@ -14,9 +14,13 @@
 // n1 < nvband with ssx in place of D.
 //
 // Kernels (one call = three launches on the caller's stream):
-//   gpp_coef_kernel   c[Kpad][ldc] from aqsm / vcoul / the column map
-//   gpp_fused_kernel  MFMA GEMM tile (LDS double-buffered) + pole/SX
-//                     epilogue + wave/workgroup reduction -> per-tile partials
+//   gpp_coef_kernel   c[Kpad][ldc] from aqsm / vcoul / the column map, with
+//                     the occupied bands first, zero-padded to a k-step
+//   gpp_fused_kernel  MFMA GEMM tile with operand fragments loaded straight
+//                     from global memory into a register ring (no LDS, no
+//                     barriers), the occupied-band partial sum taken as a
+//                     register snapshot, then the pole/SX epilogue and a
+//                     wave/workgroup reduction -> per-tile partials
 //   gpp_reduce_kernel fixed-order sum of the per-tile partials -> outputs
 // Every sum is taken in a fixed order, so results are bitwise reproducible.
 #include "fk_complex.h"
@@ -49,7 +53,11 @@ struct GppArgs {
     const int *igp;        // [G], -1 = skipped column
     double2 *partials;     // [tiles][nw][2] (achtemp part, asxtemp part)
     int nb, nv, C, G, nw, ldc, Kpad;
+    int nvpad;             // occupied bands rounded up to a k-step (4)
+    int nks;               // k-steps (Kpad / 4)
     int tiles_m, tiles_n;
+    int stagger_lo, stagger_hi;   // workgroups that start late ...
+    int stagger_ticks;            // ... by this many wall-clock ticks (100 MHz)
     double limittwo, limitone, to1;
     int mask_on;
     PoleConst pc;
@@ -62,6 +70,17 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 __device__ __forceinline__ f4 mfma(float a, float b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// -a * b + c: the FP64 MFMA's neg modifier on A (the blgp field of the
+// builtin); FP32 negates in a VALU op.
+__device__ __forceinline__ d4 mfma_nega(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
+__device__ __forceinline__ f4 mfma_nega(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(-a, b, c, 0, 0, 0);
 }
 
 template <class E> struct Acc;
@@ -138,13 +157,24 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // ---------------------------------------------------------------------------
-// c[k][m] = 0.5 vcoul[igp] conj(aqsm[k][igp]) for k < nb, m < G and a kept
-// column; zero elsewhere (padding to Kpad x ldc).  The resolved column map
-// travels in the kernel arguments (a chunk of columns per launch), so the
-// call needs no host-to-device copy; the launch also stores it for the
-// fused kernel.
+// c[kr][m] = 0.5 vcoul[igp] conj(aqsm[band][igp]) for a kept column m < G,
+// zero elsewhere.  Row kr holds band band_of(kr): the occupied bands first
+// (kr < nv), zero rows up to nvpad (a multiple of 4), then the unoccupied
+// bands; rows past the last band are zero (padding to Kpad).  The resolved
+// column map travels in the kernel arguments (a chunk of columns per
+// launch), so the call needs no host-to-device copy; the launch also stores
+// it for the fused kernel.
 // ---------------------------------------------------------------------------
 constexpr int kColChunk = 896;
+
+// band of coefficient row kr, or -1 for a zero row
+__device__ __forceinline__ int band_of(int kr, int nb, int nv, int nvpad)
+{
+    if (kr < nv) return kr;
+    if (kr < nvpad) return -1;
+    int b = kr - nvpad + nv;
+    return b < nb ? b : -1;
+}
 
 template <class E>
 struct CoefArgs {
@@ -152,7 +182,7 @@ struct CoefArgs {
     const double *vcoul;
     E *c;
     int *igp_out;
-    int nb, C, G, Kpad, ldc, m_begin, m_count;
+    int nb, nv, nvpad, C, G, Kpad, ldc, m_begin, m_count;
     int igp[kColChunk];
 };
 
@@ -162,50 +192,68 @@ __global__ void gpp_coef_kernel(CoefArgs<E> a)
     int64_t n = (int64_t)a.Kpad * a.m_count;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
-        int k = (int)(idx / a.m_count), mm = (int)(idx % a.m_count), m = a.m_begin + mm;
+        int kr = (int)(idx / a.m_count), mm = (int)(idx % a.m_count), m = a.m_begin + mm;
         int g = m < a.G ? a.igp[mm] : -1;
-        if (k == 0 && m < a.G) a.igp_out[m] = g;
+        if (kr == 0 && m < a.G) a.igp_out[m] = g;
         double2 v = make_double2(0.0, 0.0);
-        if (k < a.nb && g >= 0) {
-            double2 am = widen(a.aqsm[(size_t)k * a.C + g]);
+        int band = band_of(kr, a.nb, a.nv, a.nvpad);
+        if (band >= 0 && g >= 0) {
+            double2 am = widen(a.aqsm[(size_t)band * a.C + g]);
             double s = 0.5 * a.vcoul[g];
             v = make_double2(am.x * s, -am.y * s);
         }
-        E *dst = a.c + (size_t)k * a.ldc + m;
+        E *dst = a.c + (size_t)kr * a.ldc + m;
         if constexpr (sizeof(E) == sizeof(double2)) *dst = v;
         else *dst = make_float2((float)v.x, (float)v.y);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Fused GEMM tile + epilogue.  WGM x WGN waves per workgroup; workgroup tile
-// BM (my_igp) x BN (ig); wave tile BM/WGM x BN/WGN made of 16x16 MFMA blocks;
-// MINWG workgroups per CU; NWC frequencies per epilogue pass.
+// Fused GEMM tile + epilogue.  WGM x WGN waves per workgroup, each wave a
+// WM x WN = (16 MB) x (16 NB) tile of (my_igp, ig); MINW workgroups per CU;
+// NWC frequencies per epilogue pass.
+//
 // MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator register r of lane l is C[acc_row(l,r)][l%16].
+// With A = c[k][m] and B = aqsn[k][n] both row-major in k, a lane's fragment
+// is one 16-byte element, and the 64 lanes of a wave read four 256-byte row
+// segments: fragments are loaded straight from global memory (L1/L2) into a
+// ring of P k-steps, so the loop has no LDS traffic and no barrier.
 // Complex products use four real MFMAs (re += ar br - ai bi, im += ar bi + ai br).
 // ---------------------------------------------------------------------------
-template <class E, int BM, int BN, int BK, int WGM, int WGN, int MINWG, int NWC>
-__global__ __launch_bounds__(64 * WGM * WGN, MINWG) void gpp_fused_kernel(GppArgs<E> a)
+template <class E, int MB, int NB>
+struct Frags {
+    E a[MB], b[NB];
+};
+
+template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
+__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs<E> a)
 {
     using A4 = typename Acc<E>::type;
-    constexpr int NT = 64 * WGM * WGN, NWAVES = WGM * WGN;
-    constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
-    constexpr int LA = BK * BM / NT, LB = BK * BN / NT;
-    constexpr int KS = BK / 4;
-    static_assert(BK % 4 == 0 && WM % 16 == 0 && WN % 16 == 0, "tile shape");
-    static_assert(LA * NT == BK * BM && LB * NT == BK * BN, "tile loads");
+    constexpr int NWAVES = WGM * WGN;
+    constexpr int WM = MB * 16, WN = NB * 16, BM = WM * WGM, BN = WN * WGN;
 
-    __shared__ E As[2][BK * BM];
-    __shared__ E Bs[2][BK * BN];
     __shared__ double red[NWAVES][4 * NWC];
+    // occupied-band partial sums, one slot per (block, register, lane)
+    __shared__ E occ_lds[NWAVES][MB * NB * 4 * 64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WGM, wn = wave / WGM;
+
+    // The second workgroup slot of each CU starts about half a tile late, so
+    // that one slot's epilogue (VALU, memory) overlaps the other's MFMA loop;
+    // later workgroups inherit the offset.  The first slot runs alone at the
+    // full MFMA rate meanwhile.  The wait is bounded by the tick count.
+    if ((int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
+        const uint64_t t0 = wall_clock64();
+        while (wall_clock64() - t0 < (uint64_t)a.stagger_ticks) __builtin_amdgcn_s_sleep(32);
+    }
     int tm, tn;
     tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
-    const int kmax = a.nb - 1, nmax = a.C - 1;
+    const int rowbase = m0 + wm * WM, colbase = n0 + wn * WN;
+    const int nmax = a.C - 1;
+    const int kl = lane >> 4, cl = lane & 15;
 
     A4 acc_re[MB][NB], acc_im[MB][NB];
 #pragma unroll
@@ -215,88 +263,80 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINWG) void gpp_fused_kernel(GppArg
             acc_re[i][j] = A4{0, 0, 0, 0};
             acc_im[i][j] = A4{0, 0, 0, 0};
         }
-
-    // Global -> register staging.  c is zero-padded to Kpad x ldc, so its
-    // loads need no guard; aqsn loads read a clamped (always valid) address
-    // and select zero outside [0,nb) x [0,C), which keeps the loop free of
-    // divergent branches.
-    E ra[LA], rb[LB];
-    auto gload = [&](int k0) {
+    E *occ = &occ_lds[wave][0];
+    auto occ_store = [&](bool zero) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) {
-            int idx = tid + NT * i, kk = idx / BM, mm = idx % BM;
-            ra[i] = a.c[(size_t)(k0 + kk) * a.ldc + m0 + mm];
-        }
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int i = 0; i < LB; ++i) {
-            int idx = tid + NT * i, k = k0 + idx / BN, n = n0 + idx % BN;
-            E v = a.aqsn[(size_t)min(k, kmax) * a.C + min(n, nmax)];
-            rb[i] = (k <= kmax && n <= nmax) ? v : zero_elem<E>();
-        }
-    };
-    auto sstore = [&](int buf) {
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int i = 0; i < LA; ++i) As[buf][tid + NT * i] = ra[i];
-#pragma unroll
-        for (int i = 0; i < LB; ++i) Bs[buf][tid + NT * i] = rb[i];
+                for (int r = 0; r < 4; ++r) {
+                    E v;
+                    v.x = zero ? 0 : acc_re[i][j][r];
+                    v.y = zero ? 0 : acc_im[i][j][r];
+                    occ[((i * NB + j) * 4 + r) * 64 + lane] = v;
+                }
     };
 
-    const int nk = a.Kpad / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * BK);
+    // A fragment of k-step s, block i: c[4s + kl][rowbase + 16 i + cl]
+    // (c is zero-padded to Kpad x ldc: always in bounds).  B fragment of
+    // k-step s, block j: aqsn[band_of(4s + kl)][colbase + 16 j + cl], with the
+    // band and the column clamped into the array (a zero row of c makes the
+    // clamped value irrelevant).
+    const E *pa = a.c + (size_t)kl * a.ldc + rowbase + cl;
+    int colj[NB];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int kr = ks * 4 + (lane >> 4);
-            E af[MB], bf[NB];
+    for (int j = 0; j < NB; ++j) colj[j] = min(colbase + j * 16 + cl, nmax);
+    auto load = [&](Frags<E, MB, NB> &f, int s) {
+        s = min(s, a.nks - 1);
+        const E *qa = pa + (size_t)(4 * s) * a.ldc;
 #pragma unroll
-            for (int i = 0; i < MB; ++i) af[i] = As[buf][kr * BM + wm * WM + i * 16 + (lane & 15)];
+        for (int i = 0; i < MB; ++i) f.a[i] = qa[i * 16];
+        const int kr = 4 * s + kl;
+        int band = kr < a.nv ? kr : kr - a.nvpad + a.nv;
+        band = min(max(band, 0), a.nb - 1);
+        const E *qb = a.aqsn + (size_t)band * a.C;
 #pragma unroll
-            for (int j = 0; j < NB; ++j) bf[j] = Bs[buf][kr * BN + wn * WN + j * 16 + (lane & 15)];
+        for (int j = 0; j < NB; ++j) f.b[j] = qb[colj[j]];
+    };
+    auto mma = [&](const Frags<E, MB, NB> &f) {
 #pragma unroll
-            for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    acc_re[i][j] = mfma(af[i].x, bf[j].x, acc_re[i][j]);
-                    acc_im[i][j] = mfma(af[i].x, bf[j].y, acc_im[i][j]);
-                }
+            for (int j = 0; j < NB; ++j) {
+                acc_re[i][j] = mfma(f.a[i].x, f.b[j].x, acc_re[i][j]);
+                acc_im[i][j] = mfma(f.a[i].x, f.b[j].y, acc_im[i][j]);
+            }
 #pragma unroll
-            for (int i = 0; i < MB; ++i)
+        for (int i = 0; i < MB; ++i)
 #pragma unroll
-                for (int j = 0; j < NB; ++j) {
-                    acc_re[i][j] = mfma(-af[i].y, bf[j].y, acc_re[i][j]);
-                    acc_im[i][j] = mfma(af[i].y, bf[j].x, acc_im[i][j]);
-                }
+            for (int j = 0; j < NB; ++j) {
+                acc_re[i][j] = mfma_nega(f.a[i].y, f.b[j].y, acc_re[i][j]);
+                acc_im[i][j] = mfma(f.a[i].y, f.b[j].x, acc_im[i][j]);
+            }
+    };
+
+    // k-steps of the occupied bands come first; after the last of them the
+    // accumulators hold the occupied-band contraction (asxtemp's factor)
+    const int s_occ = a.nvpad / 4;
+    if (s_occ == 0) occ_store(true);
+    Frags<E, MB, NB> ring[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) load(ring[q], q);
+    for (int s0 = 0; s0 < a.nks; s0 += P) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            mma(ring[q]);
+            if (s0 + q + 1 == s_occ) occ_store(false);
+            load(ring[q], s0 + q + P);
         }
-        if (kt + 1 < nk) sstore(buf ^ 1);
-        __syncthreads();
     }
 
     // ---- epilogue ----------------------------------------------------------
-    // Per (my_igp, ig) of the tile: B from the accumulators, Bocc (the same
-    // contraction over the occupied bands n1 < nv) from a few MFMAs on the
-    // occupied bands staged in LDS, then per frequency the pole factor D and
-    // the SX factor ssx, accumulated per lane and reduced over the tile.
+    // Per (my_igp, ig) of the tile: B and Bocc from the registers, then per
+    // frequency the pole factor D and the SX factor ssx, accumulated per lane
+    // and reduced over the tile.
     const bool want_ssx = a.nv > 0;
-    const int nvpad = (a.nv + 3) & ~3;
-    const bool occ_lds = nvpad <= 2 * BK;      // both LDS stages hold the occupied bands
-    E *Aocc = &As[0][0], *Bocc_s = &Bs[0][0];   // [2*BK][BM] and [2*BK][BN], contiguous
-    if (want_ssx && occ_lds) {
-        for (int idx = tid; idx < nvpad * BM; idx += NT) {
-            int k = idx / BM, mm = idx % BM;
-            E v = a.c[(size_t)k * a.ldc + m0 + mm];
-            Aocc[idx] = k < a.nv ? v : zero_elem<E>();
-        }
-        for (int idx = tid; idx < nvpad * BN; idx += NT) {
-            int k = idx / BN, n = n0 + idx % BN;
-            E v = a.aqsn[(size_t)min(k, a.nv - 1) * a.C + min(n, nmax)];
-            Bocc_s[idx] = (k < a.nv && n <= nmax) ? v : zero_elem<E>();
-        }
-    }
-    __syncthreads();
 
     // rows of this lane that are real, kept columns
     unsigned rowok = 0;
@@ -304,7 +344,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINWG) void gpp_fused_kernel(GppArg
     for (int i = 0; i < MB; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = m0 + wm * WM + i * 16 + acc_row<E>(lane, r);
+            const int row = rowbase + i * 16 + acc_row<E>(lane, r);
             const bool ok = row < a.G && a.igp[min(row, a.G - 1)] >= 0;
             rowok |= (ok ? 1u : 0u) << (i * 4 + r);
         }
@@ -319,34 +359,22 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINWG) void gpp_fused_kernel(GppArg
         for (int i = 0; i < MB; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                A4 oc_re = A4{0, 0, 0, 0}, oc_im = A4{0, 0, 0, 0};
-                if (want_ssx && occ_lds) {
-                    for (int ks = 0; ks < nvpad / 4; ++ks) {
-                        const int kr = ks * 4 + (lane >> 4);
-                        E af = Aocc[kr * BM + wm * WM + i * 16 + (lane & 15)];
-                        E bf = Bocc_s[kr * BN + wn * WN + j * 16 + (lane & 15)];
-                        oc_re = mfma(af.x, bf.x, oc_re);
-                        oc_im = mfma(af.x, bf.y, oc_im);
-                        oc_re = mfma(-af.y, bf.y, oc_re);
-                        oc_im = mfma(af.y, bf.x, oc_im);
-                    }
-                }
-                const int col = n0 + wn * WN + j * 16 + (lane & 15);
+                const int col = colbase + j * 16 + cl;
+                E cwt[4], cie[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = m0 + wm * WM + i * 16 + acc_row<E>(lane, r);
-                    const bool ok = ((rowok >> (i * 4 + r)) & 1u) && col <= nmax;
+                    const int row = rowbase + i * 16 + acc_row<E>(lane, r);
                     const size_t e = (size_t)min(row, a.G - 1) * a.C + min(col, nmax);
-                    const double2 wt = widen(a.wtilde[e]);
-                    const double2 ie = widen(a.I_eps[e]);
+                    cwt[r] = a.wtilde[e];
+                    cie[r] = a.I_eps[e];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const bool ok = ((rowok >> (i * 4 + r)) & 1u) && col <= nmax;
+                    const double2 wt = widen(cwt[r]);
+                    const double2 ie = widen(cie[r]);
                     const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
-                    double2 Bo = make_double2((double)oc_re[r], (double)oc_im[r]);
-                    if (want_ssx && !occ_lds) {
-                        Bo = make_double2(0.0, 0.0);
-                        for (int n1 = 0; n1 < a.nv; ++n1)
-                            Bo = cfma(Bo, widen(a.c[(size_t)n1 * a.ldc + min(row, a.ldc - 1)]),
-                                      widen(a.aqsn[(size_t)n1 * a.C + min(col, nmax)]));
-                    }
+                    const double2 Bo = widen(occ[((i * NB + j) * 4 + r) * 64 + lane]);
                     const double2 wt2 = cmul(wt, wt);
                     const double2 Om = cmul(wt2, ie);
 #pragma unroll
@@ -448,23 +476,23 @@ __global__ __launch_bounds__(256) void gpp_reduce_kernel(const double2 *partials
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-struct TileCfg { int bm, bn, bk; };
-// config ids (fk_options.config = id + 1):
-//   0: 64x128, BK 16, 2x2 waves, 1 workgroup/CU
-//   1: 32x64,  BK 16, 2x2 waves, 1 workgroup/CU   (small problems)
-//   2: 64x128, BK 8,  2x2 waves, 2 workgroups/CU
-//   3: 64x128, BK 16, 2x4 waves, 1 workgroup/CU
-constexpr TileCfg kCfgs[] = {{64, 128, 16}, {32, 64, 16}, {64, 128, 8}, {64, 128, 16}};
+struct TileCfg { int bm, bn, p, minw; };
+// config ids (fk_options.config = id + 1): wave tile, ring depth, waves, WG/CU
+//   0: 32x64 waves, ring 4, 1x4 waves (32x256 tile), 1 workgroup/CU
+//   1: 32x32 waves, ring 3, 2x2 waves (64x64 tile),  2 workgroups/CU
+//   2: 64x32 waves, ring 3, 2x2 waves (128x64 tile), 2 workgroups/CU
+//   3: 32x64 waves, ring 3, 2x2 waves (64x128 tile), 2 workgroups/CU
+constexpr TileCfg kCfgs[] = {{32, 256, 4, 1}, {64, 64, 3, 2}, {128, 64, 3, 2}, {64, 128, 3, 2}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 template <class E, int NWC>
 void launch_fused_nw(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 {
     switch (cfg) {
-    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 16, 2, 2, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((gpp_fused_kernel<E, 32, 64, 16, 2, 2, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 8, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 64, 128, 16, 2, 4, 1, NWC>), dim3(tiles), dim3(512), 0, s, a); break;
+    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 4, 1, 4, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 2, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((gpp_fused_kernel<E, 4, 2, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
     }
 }
 
@@ -482,15 +510,15 @@ void launch_fused(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 
 int choose_cfg(const fk_gpp_problem *p, const fk_options *o)
 {
+    (void)p;
     if (o->config > 0) return o->config - 1;
-    // 64x128 tiles, two workgroups per CU, unless that leaves the 256 CUs
-    // with under two tiles each; then the 32x64 tile
-    long tiles = (long)div_up(p->ngpown, kCfgs[2].bm) * div_up(p->ncouls, kCfgs[2].bn);
-    return tiles >= 512 ? 2 : 1;
+    // 32x32 wave tiles, two staggered workgroups per CU: the fastest measured
+    // at 512/2/32768/20 and 128/2/8192/20
+    return 1;
 }
 
 struct Layout {
-    int cfg, Kpad, ldc, tiles_m, tiles_n;
+    int cfg, Kpad, nvpad, ldc, tiles_m, tiles_n;
     size_t off_c, off_igp, off_part;
 };
 
@@ -499,7 +527,9 @@ Layout plan_layout(const fk_gpp_problem *p, const fk_options *o, Scratch &ws, si
     Layout L;
     L.cfg = choose_cfg(p, o);
     const TileCfg &t = kCfgs[L.cfg];
-    L.Kpad = div_up(std::max(p->number_bands, 1), t.bk) * t.bk;
+    L.nvpad = div_up(p->nvband, 4) * 4;
+    const int rows = std::max(L.nvpad + p->number_bands - p->nvband, 1);
+    L.Kpad = div_up(rows, 4 * t.p) * 4 * t.p;
     L.ldc = div_up(p->ngpown, t.bm) * t.bm;
     L.tiles_m = L.ldc / t.bm;
     L.tiles_n = div_up(p->ncouls, t.bn);
@@ -549,7 +579,8 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     for (int m_begin = 0; m_begin < L.ldc; m_begin += kColChunk) {
         CoefArgs<E> ca;
         ca.aqsm = aqsm; ca.vcoul = vcoul; ca.c = c; ca.igp_out = igp;
-        ca.nb = p->number_bands; ca.C = C; ca.G = G; ca.Kpad = L.Kpad; ca.ldc = L.ldc;
+        ca.nb = p->number_bands; ca.nv = p->nvband; ca.nvpad = L.nvpad;
+        ca.C = C; ca.G = G; ca.Kpad = L.Kpad; ca.ldc = L.ldc;
         ca.m_begin = m_begin;
         ca.m_count = std::min(kColChunk, L.ldc - m_begin);
         for (int mm = 0; mm < kColChunk; ++mm) {
@@ -565,7 +596,26 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     GppArgs<E> a;
     a.c = c; a.aqsn = aqsn; a.I_eps = ieps; a.wtilde = wt; a.igp = igp; a.partials = part;
     a.nb = p->number_bands; a.nv = p->nvband; a.C = C; a.G = G; a.nw = p->nfreq; a.ldc = L.ldc; a.Kpad = L.Kpad;
+    a.nvpad = L.nvpad; a.nks = L.Kpad / 4;
     a.tiles_m = L.tiles_m; a.tiles_n = L.tiles_n;
+    const int tiles = L.tiles_m * L.tiles_n;
+    a.stagger_lo = a.stagger_hi = 0;
+    a.stagger_ticks = 0;
+    if (kCfgs[L.cfg].minw == 2) {
+        int dev = 0, ncu = 0;
+        FK_HIP(hipGetDevice(&dev));
+        FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (tiles > 2 * ncu) {
+            // half a tile of MFMA work at half a CU's rate (two workgroups
+            // share it): FP64 MFMA 128 flop/cycle/CU, FP32 256; 2.4 GHz clock,
+            // 100 MHz wall clock
+            const double cu_rate = sizeof(E) == sizeof(double2) ? 128.0 : 256.0;
+            const double tile_cycles = 8.0 * kCfgs[L.cfg].bm * kCfgs[L.cfg].bn * L.Kpad / (cu_rate / 2);
+            a.stagger_lo = ncu;
+            a.stagger_hi = 2 * ncu;
+            a.stagger_ticks = (int)std::min(0.5 * tile_cycles / 24.0, 1.0e6);
+        }
+    }
     a.limittwo = p->limittwo; a.limitone = p->limitone; a.to1 = p->to1; a.mask_on = o->mask ? 1 : 0;
     for (int iw = 0; iw < kMaxFreq; ++iw) {
         double w = iw < p->nfreq ? p->wx[iw] : 0.0;
@@ -574,7 +624,6 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
         a.pc.wx2[iw] = w2;
         a.pc.rden1[iw] = 1.0 / (w2 * w2);
     }
-    const int tiles = L.tiles_m * L.tiles_n;
     KernelTimer timer(o->kernel_ms, s);
     FK_TRY(timer.start());
     launch_fused<E>(L.cfg, tiles, s, a);
