@@ -242,6 +242,16 @@ def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index
     return schDt, schDv, achs, (float(ms.value) if time_kernel else None)
 
 
+def ff_workspace_size(nb, ncouls, ngpown, nfreq, config=0):
+    dummy = 1  # non-NULL placeholders: only sizes are read
+    inv = np.zeros(ngpown, np.int32)
+    ind = np.zeros(ncouls + 1, np.int32)
+    prob = FfProblem(nb, ncouls, ngpown, nfreq, dummy, dummy, dummy, dummy, inv.ctypes.data, ind.ctypes.data, 0)
+    opt = default_options()
+    opt.config = config
+    return int(lib.ff_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
+
+
 def ff_problem(p, **kw):
     return ff_kernel(p.nb, p.ncouls, p.ngpown, p.nfreq, p.aqsm, p.aqsn, p.I_eps, p.vcoul, p.inv_igp_index,
                      p.indinv, **kw)

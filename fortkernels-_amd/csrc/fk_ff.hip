@@ -1,16 +1,471 @@
-// fk_ff.hip -- FF frequency contraction (placeholder until the MFMA kernel lands).
+// fk_ff.hip -- FF frequency contraction on MI355X (gfx950).
+//
+// The Fortran original contracts (ffkernel.deslippe.best.f90.back1:402-418)
+//     tempz(my_igp, ifreq, n1) = sum_ig I_epsR(ig, my_igp, ifreq) aqsntemp(ig, n1)
+//         (zgemm('t','n', ngpown*nfreq, number_bands, ncouls, ...), :404)
+//     schdt_matrix(ifreq, n1) += sum_my_igp tempz(my_igp, ifreq, n1) conj(aqsmtemp(igp, n1))
+// and the static CH sum (ffkernel.deslippe.best.f90:251-267; C++
+// FFKer_cori/ffKernel.cpp:107-131) is the vcoul-weighted contraction at
+// one frequency.  In C order, with X = I_eps viewed as [nfreq*ngpown][ncouls]:
+//     T[n1][m] = sum_ig aqsn[n1][ig] X[m][ig]            (GEMM, K = ncouls)
+//     schDt[n1][f] = sum_my_igp cf[n1][my_igp] T[n1][f*G + my_igp]
+//     schDv[n1][f] = sum_my_igp vg[my_igp] cf[n1][my_igp] T[n1][f*G + my_igp]
+//     achsDtemp   = -0.5 sum_n1 schDv[n1][achs_freq]
+// cf[n1][my_igp] = conj(aqsm[n1][igp]) and vg[my_igp] = vcoul[igp] (zero for a
+// skipped column).  tempz is never stored.
+//
+// Kernels: ff_coef_kernel (cf, vg), ff_fused_kernel (MFMA GEMM tile with
+// fragments loaded straight from global memory + row contraction of the
+// tile -> per-tile partials), ff_reduce_kernel (fixed-order sums).
+#include "fk_complex.h"
 #include "fk_internal.h"
+
+#include <algorithm>
+
+namespace fk {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kXcds = 8;
+
+struct FfArgs {
+    const double2 *aqsn;    // [nb][C]
+    const double2 *X;       // [nfreq*G][C]
+    const double2 *cf;      // [nb][G]  conj(aqsm[n1][igp])
+    const double *vg;       // [G]      vcoul[igp]
+    double2 *partials;      // [nfreq][mtiles][nb][2] (schDt part, schDv part)
+    int nb, C, G, nfreq, nks;
+    int tiles_m, tiles_n, mtiles;   // tiles_n = nfreq * mtiles
+    int stagger_lo, stagger_hi, stagger_ticks;
+};
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ d4 mfma_nega(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
+
+// Workgroup b -> tile (tm over bands fastest within an XCD's range, so the
+// band tiles that share one block of X run back to back on one XCD).
+__device__ __forceinline__ void tile_coords(int b, int tiles_m, int total, int &tm, int &tn)
+{
+    int q = total / kXcds, r = total % kXcds, x = b % kXcds, i = b / kXcds;
+    int t = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    tm = t % tiles_m;
+    tn = t / tiles_m;
+}
+
+__global__ void ff_coef_kernel(const double2 *aqsm, const double *vcoul, const int *igp_in, int nb, int C, int G,
+                               double2 *cf, double *vg)
+{
+    int64_t n = (int64_t)nb * G;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
+        int n1 = (int)(idx / G), m = (int)(idx % G);
+        int g = igp_in[m];
+        double2 v = make_double2(0.0, 0.0);
+        if (g >= 0) {
+            double2 am = aqsm[(size_t)n1 * C + g];
+            v = make_double2(am.x, -am.y);
+        }
+        cf[idx] = v;
+        if (n1 == 0) vg[m] = g >= 0 ? vcoul[g] : 0.0;
+    }
+}
+
+template <int MB, int NB>
+struct Frags {
+    double2 a[MB], b[NB];
+};
+
+// 16x16x4 FP64 MFMA: lane l supplies A[l%16][l/16] (here aqsn[n1 = row l%16]
+// [ig = l/16]) and B[l/16][l%16] (X[m = col l%16][ig = l/16]); accumulator
+// register r of lane l holds T[4r + l/16][l%16].
+template <int MB, int NB, int P, int WGM, int WGN, int MINW>
+__global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a)
+{
+    constexpr int NWAVES = WGM * WGN;
+    constexpr int WM = MB * 16, WN = NB * 16, BM = WM * WGM, BN = WN * WGN;
+    __shared__ double2 red[NWAVES][2 * WM];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WGM, wn = wave / WGM;
+    if ((int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
+        const uint64_t t0 = wall_clock64();
+        while (wall_clock64() - t0 < (uint64_t)a.stagger_ticks) __builtin_amdgcn_s_sleep(32);
+    }
+    int tm, tn;
+    tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
+    const int f = tn / a.mtiles, mt = tn % a.mtiles;
+    const int rowbase = tm * BM + wm * WM;            // band n1
+    const int colbase = mt * BN + wn * WN;            // my_igp within frequency f
+    const int kl = lane >> 4, cl = lane & 15;
+
+    d4 acc_re[MB][NB], acc_im[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            acc_re[i][j] = d4{0, 0, 0, 0};
+            acc_im[i][j] = d4{0, 0, 0, 0};
+        }
+
+    // row / column addresses clamped into the arrays; rows past nb and
+    // columns past G are dropped in the epilogue, k past C is zeroed
+    const double2 *pa[MB];
+    const double2 *pb[NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) pa[i] = a.aqsn + (size_t)min(rowbase + i * 16 + cl, a.nb - 1) * a.C;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        pb[j] = a.X + ((size_t)f * a.G + min(colbase + j * 16 + cl, a.G - 1)) * a.C;
+    auto load = [&](Frags<MB, NB> &fr, int s) {
+        s = min(s, a.nks - 1);
+        const int k = 4 * s + kl;
+        const int kc = min(k, a.C - 1);
+        const bool kok = k < a.C;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            double2 v = pa[i][kc];
+            fr.a[i] = kok ? v : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) fr.b[j] = pb[j][kc];
+    };
+    auto mma = [&](const Frags<MB, NB> &fr) {
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                acc_re[i][j] = mfma(fr.a[i].x, fr.b[j].x, acc_re[i][j]);
+                acc_im[i][j] = mfma(fr.a[i].x, fr.b[j].y, acc_im[i][j]);
+            }
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                acc_re[i][j] = mfma_nega(fr.a[i].y, fr.b[j].y, acc_re[i][j]);
+                acc_im[i][j] = mfma(fr.a[i].y, fr.b[j].x, acc_im[i][j]);
+            }
+    };
+    Frags<MB, NB> ring[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) load(ring[q], q);
+    for (int s0 = 0; s0 < a.nks; s0 += P) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            mma(ring[q]);
+            load(ring[q], s0 + q + P);
+        }
+    }
+
+    // ---- epilogue: contract each row with cf (and vg cf) over the tile's columns
+    double2 rt[MB][4], rv[MB][4];     // per row (i, r): schDt and schDv partials
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { rt[i][r] = make_double2(0.0, 0.0); rv[i][r] = make_double2(0.0, 0.0); }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int col = colbase + j * 16 + cl;
+        const bool cok = col < a.G;
+        const double vgc = a.vg[min(col, a.G - 1)];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rowbase + i * 16 + 4 * r + kl;
+                const double2 c = a.cf[(size_t)min(row, a.nb - 1) * a.G + min(col, a.G - 1)];
+                const double2 p = cmul(make_double2(acc_re[i][j][r], acc_im[i][j][r]), c);
+                if (cok) {
+                    rt[i][r] = cadd(rt[i][r], p);
+                    rv[i][r] = cadd(rv[i][r], make_double2(p.x * vgc, p.y * vgc));
+                }
+            }
+    }
+    // sum over the 16 lanes that hold one row (fixed xor order), then over
+    // the waves of the workgroup that share the rows (fixed order)
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                rt[i][r].x += __shfl_xor(rt[i][r].x, off, 64);
+                rt[i][r].y += __shfl_xor(rt[i][r].y, off, 64);
+                rv[i][r].x += __shfl_xor(rv[i][r].x, off, 64);
+                rv[i][r].y += __shfl_xor(rv[i][r].y, off, 64);
+            }
+        }
+    for (int w = 0; w < WGN; ++w) {
+        if (wn == w && cl == 0) {
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int lr = i * 16 + 4 * r + kl;    // row within the wave tile
+                    double2 *dt = &red[wm][2 * lr];
+                    if (w == 0) { dt[0] = rt[i][r]; dt[1] = rv[i][r]; }
+                    else { dt[0] = cadd(dt[0], rt[i][r]); dt[1] = cadd(dt[1], rv[i][r]); }
+                }
+        }
+        __syncthreads();
+    }
+    // one row per thread: write this tile's partial for (f, mt, n1)
+    for (int t = tid; t < BM; t += 64 * NWAVES) {
+        const int w = t / WM, lr = t % WM;
+        const int n1 = tm * BM + t;
+        if (n1 < a.nb) {
+            double2 *dst = a.partials + (((size_t)f * a.mtiles + mt) * a.nb + n1) * 2;
+            dst[0] = red[w][2 * lr];
+            dst[1] = red[w][2 * lr + 1];
+        }
+    }
+}
+
+// schDt[n1][f], schDv[n1][f]: sum over the column tiles in order
+__global__ void ff_reduce_kernel(const double2 *partials, int nb, int nfreq, int mtiles, double2 *schDt,
+                                 double2 *schDv)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * nfreq) return;
+    const int n1 = idx / nfreq, f = idx % nfreq;
+    double2 st = make_double2(0.0, 0.0), sv = st;
+    for (int mt = 0; mt < mtiles; ++mt) {
+        const double2 *p = partials + (((size_t)f * mtiles + mt) * nb + n1) * 2;
+        st = cadd(st, p[0]);
+        sv = cadd(sv, p[1]);
+    }
+    schDt[idx] = st;
+    if (schDv) schDv[idx] = sv;
+}
+
+// achsDtemp = -0.5 sum_n1 schDv[n1][achs_freq], one workgroup, fixed order
+__global__ __launch_bounds__(256) void ff_achs_kernel(const double2 *partials, int nb, int nfreq, int mtiles,
+                                                       int achs_freq, double2 *achs)
+{
+    __shared__ double sre[256], sim[256];
+    double re = 0.0, im = 0.0;
+    for (int n1 = threadIdx.x; n1 < nb; n1 += 256) {
+        for (int mt = 0; mt < mtiles; ++mt) {
+            const double2 v = partials[(((size_t)achs_freq * mtiles + mt) * nb + n1) * 2 + 1];
+            re += v.x;
+            im += v.y;
+        }
+    }
+    sre[threadIdx.x] = re;
+    sim[threadIdx.x] = im;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sre[threadIdx.x] += sre[threadIdx.x + w];
+            sim[threadIdx.x] += sim[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *achs = make_double2(-0.5 * sre[0], -0.5 * sim[0]);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct FfCfg { int bm, bn, p, minw; };
+// config ids (fk_options.config = id + 1)
+//   0: 32x32 waves, 2x2 waves (64x64 tile), ring 3, 2 workgroups/CU
+//   1: 32x64 waves, 2x2 waves (64x128 tile), ring 3, 1 workgroup/CU (default:
+//      15.4 ms against 16.2 ms for id 0 at 256/4096/4096/32 on MI355X)
+constexpr FfCfg kFfCfgs[] = {{64, 64, 3, 2}, {64, 128, 3, 1}};
+constexpr int kFfDefaultCfg = 1;
+constexpr int kNumFfCfgs = sizeof(kFfCfgs) / sizeof(kFfCfgs[0]);
+
+void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
+{
+    switch (cfg) {
+    case 0: hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2>), dim3(tiles), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1>), dim3(tiles), dim3(256), 0, s, a); break;
+    }
+}
+
+struct FfLayout {
+    int cfg, tiles_m, mtiles, nks;
+    size_t off_cf, off_vg, off_igp, off_part;
+};
+
+FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
+{
+    FfLayout L;
+    L.cfg = o->config > 0 ? o->config - 1 : kFfDefaultCfg;
+    const FfCfg &c = kFfCfgs[L.cfg];
+    L.tiles_m = div_up(std::max(p->number_bands, 1), c.bm);
+    L.mtiles = div_up(p->ngpown, c.bn);
+    L.nks = div_up(p->ncouls, 4 * c.p) * c.p;
+    L.off_cf = ws.plan(sizeof(double2) * (size_t)std::max(p->number_bands, 1) * p->ngpown);
+    L.off_vg = ws.plan(sizeof(double) * (size_t)p->ngpown);
+    L.off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
+    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * L.mtiles * std::max(p->number_bands, 1));
+    return L;
+}
+
+int ff_check(const fk_ff_problem *p, const fk_options *o)
+{
+    if (!p || !o) { set_error("ff_kernel: NULL problem or options"); return FK_ERR_ARG; }
+    if (p->number_bands < 1 || p->ncouls < 1 || p->ngpown < 1 || p->nfreq < 1) {
+        set_error("ff_kernel: bad sizes (nb=%d ncouls=%d ngpown=%d nfreq=%d; all must be >= 1)", p->number_bands,
+                  p->ncouls, p->ngpown, p->nfreq);
+        return FK_ERR_ARG;
+    }
+    if (p->ngpown > p->ncouls) {   // FFKer_cori/ffKernel.cpp:320-325
+        set_error("ff_kernel: ngpown (%d) cannot be greater than ncouls (%d)", p->ngpown, p->ncouls);
+        return FK_ERR_ARG;
+    }
+    if (!p->aqsmtemp || !p->aqsntemp || !p->I_eps_array || !p->vcoul || !p->inv_igp_index || !p->indinv) {
+        set_error("ff_kernel: NULL input array");
+        return FK_ERR_ARG;
+    }
+    if (p->achs_freq < 0 || p->achs_freq >= p->nfreq) {
+        set_error("ff_kernel: achs_freq %d outside [0, nfreq=%d)", p->achs_freq, p->nfreq);
+        return FK_ERR_ARG;
+    }
+    if (o->memory != FK_MEM_HOST && o->memory != FK_MEM_DEVICE) { set_error("ff_kernel: bad memory kind"); return FK_ERR_ARG; }
+    if (o->precision != FK_PREC_FP64) { set_error("ff_kernel: only FK_PREC_FP64 in this build"); return FK_ERR_UNSUPPORTED; }
+    if (o->config < 0 || o->config > kNumFfCfgs) { set_error("ff_kernel: config %d out of range", o->config); return FK_ERR_ARG; }
+    return FK_OK;
+}
+
+int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
+              const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, hipStream_t s)
+{
+    const int nb = p->number_bands, C = p->ncouls, G = p->ngpown, nf = p->nfreq;
+    double2 *cf = static_cast<double2 *>(ws.at(L.off_cf));
+    double *vg = static_cast<double *>(ws.at(L.off_vg));
+    int *igp = static_cast<int *>(ws.at(L.off_igp));
+    double2 *part = static_cast<double2 *>(ws.at(L.off_part));
+    FK_HIP(hipMemcpyAsync(igp, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, s));
+    {
+        int64_t n = (int64_t)nb * G;
+        int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(ff_coef_kernel, dim3(blocks), dim3(256), 0, s, aqsm, vcoul, igp, nb, C, G, cf, vg);
+        FK_HIP(hipGetLastError());
+    }
+    FfArgs a;
+    a.aqsn = aqsn; a.X = X; a.cf = cf; a.vg = vg; a.partials = part;
+    a.nb = nb; a.C = C; a.G = G; a.nfreq = nf; a.nks = L.nks;
+    a.tiles_m = L.tiles_m; a.mtiles = L.mtiles; a.tiles_n = nf * L.mtiles;
+    const int tiles = a.tiles_m * a.tiles_n;
+    a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
+    if (kFfCfgs[L.cfg].minw == 2) {
+        int dev = 0, ncu = 0;
+        FK_HIP(hipGetDevice(&dev));
+        FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (tiles > 2 * ncu) {
+            const double tile_cycles = 8.0 * kFfCfgs[L.cfg].bm * kFfCfgs[L.cfg].bn * 4.0 * L.nks / 64.0;
+            a.stagger_lo = ncu;
+            a.stagger_hi = 2 * ncu;
+            a.stagger_ticks = (int)std::min(0.5 * tile_cycles / 24.0, 1.0e6);
+        }
+    }
+    KernelTimer timer(o->kernel_ms, s);
+    FK_TRY(timer.start());
+    launch_ff_fused(L.cfg, tiles, s, a);
+    FK_HIP(hipGetLastError());
+    FK_TRY(timer.stop());
+    hipLaunchKernelGGL(ff_reduce_kernel, dim3(div_up(nb * nf, 256)), dim3(256), 0, s, part, nb, nf, L.mtiles, schDt,
+                       schDv);
+    FK_HIP(hipGetLastError());
+    if (achs) {
+        hipLaunchKernelGGL(ff_achs_kernel, dim3(1), dim3(256), 0, s, part, nb, nf, L.mtiles, p->achs_freq, achs);
+        FK_HIP(hipGetLastError());
+    }
+    return timer.finish();
+}
+
+}  // namespace
+}  // namespace fk
 
 using namespace fk;
 
 extern "C" {
 
-size_t ff_kernel_workspace_size(const fk_ff_problem *, const fk_options *) { return 0; }
-
-int ff_kernel(const fk_ff_problem *, const fk_options *, fk_cdouble *, fk_cdouble *, fk_cdouble *)
+size_t ff_kernel_workspace_size(const fk_ff_problem *p, const fk_options *o)
 {
-    set_error("ff_kernel: not available in this build");
-    return FK_ERR_UNSUPPORTED;
+    fk_options d = fk_default_options();
+    if (!o) o = &d;
+    if (ff_check(p, o) != FK_OK) return 0;
+    Scratch ws;
+    ff_plan(p, o, ws);
+    return ws.total();
+}
+
+int ff_kernel(const fk_ff_problem *p, const fk_options *o, fk_cdouble *schDt, fk_cdouble *schDv,
+              fk_cdouble *achsDtemp)
+{
+    clear_error();
+    FK_TRY(ff_check(p, o));
+    if (!schDt) { set_error("ff_kernel: NULL schDt"); return FK_ERR_ARG; }
+    std::vector<int32_t> cols;
+    FK_TRY(resolve_columns(p->ncouls, p->ngpown, p->inv_igp_index, p->indinv, cols));
+    int prev_dev = -1;
+    if (o->memory == FK_MEM_HOST && o->device >= 0) {
+        FK_HIP(hipGetDevice(&prev_dev));
+        FK_HIP(hipSetDevice(o->device));
+    }
+    hipStream_t s = static_cast<hipStream_t>(o->stream);
+    const size_t nbC = (size_t)p->number_bands * p->ncouls;
+    const size_t XC = (size_t)p->nfreq * p->ngpown * p->ncouls;
+    const size_t nout = (size_t)p->number_bands * p->nfreq;
+    int rc = FK_OK;
+    {
+        Scratch ws;
+        FfLayout L = ff_plan(p, o, ws);
+        rc = ws.acquire(o->workspace, o->workspace_bytes, s);
+        if (rc == FK_OK && o->memory == FK_MEM_DEVICE) {
+            rc = ff_launch(p, o, L, ws, cols, (const double2 *)p->aqsmtemp, (const double2 *)p->aqsntemp,
+                           (const double2 *)p->I_eps_array, p->vcoul, (double2 *)schDt, (double2 *)schDv,
+                           (double2 *)achsDtemp, s);
+        } else if (rc == FK_OK) {
+            double2 *dm = nullptr, *dn = nullptr, *dx = nullptr, *dout = nullptr;
+            double *dv = nullptr;
+            auto alloc = [&](void **ptr, size_t bytes) {
+                if (rc != FK_OK) return;
+                if (hipMalloc(ptr, bytes) != hipSuccess) { set_error("hipMalloc(%zu) failed", bytes); rc = FK_ERR_NOMEM; }
+            };
+            auto up = [&](void *dst, const void *src, size_t bytes) {
+                if (rc != FK_OK) return;
+                hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+                if (e != hipSuccess) { set_error("upload: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
+            };
+            alloc((void **)&dm, nbC * sizeof(double2));
+            alloc((void **)&dn, nbC * sizeof(double2));
+            alloc((void **)&dx, XC * sizeof(double2));
+            alloc((void **)&dv, sizeof(double) * p->ncouls);
+            alloc((void **)&dout, sizeof(double2) * (2 * nout + 1));
+            up(dm, p->aqsmtemp, nbC * sizeof(double2));
+            up(dn, p->aqsntemp, nbC * sizeof(double2));
+            up(dx, p->I_eps_array, XC * sizeof(double2));
+            up(dv, p->vcoul, sizeof(double) * p->ncouls);
+            if (rc == FK_OK)
+                rc = ff_launch(p, o, L, ws, cols, dm, dn, dx, dv, dout, dout + nout, dout + 2 * nout, s);
+            if (rc == FK_OK) {
+                hipError_t e = hipMemcpyAsync(schDt, dout, sizeof(double2) * nout, hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess && schDv)
+                    e = hipMemcpyAsync(schDv, dout + nout, sizeof(double2) * nout, hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess && achsDtemp)
+                    e = hipMemcpyAsync(achsDtemp, dout + 2 * nout, sizeof(double2), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) { set_error("download: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
+            } else {
+                (void)hipStreamSynchronize(s);
+            }
+            for (void *ptr : {(void *)dm, (void *)dn, (void *)dx, (void *)dv, (void *)dout})
+                if (ptr) (void)hipFree(ptr);
+        }
+    }
+    if (prev_dev >= 0) (void)hipSetDevice(prev_dev);
+    return rc;
 }
 
 }  // extern "C"

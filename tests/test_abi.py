@@ -98,3 +98,31 @@ def test_complex_helpers_host(fk):
     assert np.allclose(ap("conj", a=ap("mul", a=a, b=b)), ap("mul", a=np.conj(a), b=np.conj(b)), rtol=1e-15)
     assert np.allclose(ap("div", a=a, b=a), np.ones(64), rtol=1e-15, atol=1e-15)
     assert np.allclose(ap("mul", a=a, b=np.conj(a)).real, ap("abs2", a=a).real, rtol=1e-15)
+
+
+def test_ff_rejects_bad_problems_before_touching_a_device(fk, fko):
+    """ngpown > ncouls is the reference program's usage error (FFKer_cori/ffKernel.cpp:320-325);
+    achs_freq must name one of the frequencies; sizes must be positive."""
+    p = fko.ff_random(4, 32, 8, 2, seed=1)
+    args = (p.aqsm, p.aqsn, p.I_eps, p.vcoul, p.inv_igp_index, p.indinv)
+    with pytest.raises(fk.FkError) as e:
+        fk.ff_kernel(p.nb, 8, 32, p.nfreq, *args)
+    assert e.value.code == fk.FK_ERR_ARG and "cannot be greater" in fk.last_error()
+    for kw in ({"achs_freq": 2}, {"achs_freq": -1}):
+        with pytest.raises(fk.FkError) as e:
+            fk.ff_kernel(p.nb, p.ncouls, p.ngpown, p.nfreq, *args, **kw)
+        assert e.value.code == fk.FK_ERR_ARG and "achs_freq" in fk.last_error()
+    with pytest.raises(fk.FkError) as e:
+        fk.ff_kernel(0, p.ncouls, p.ngpown, p.nfreq, *args)
+    assert e.value.code == fk.FK_ERR_ARG
+    bad = p.inv_igp_index.copy()
+    bad[0] = p.ncouls + 1
+    with pytest.raises(fk.FkError) as e:
+        fk.ff_kernel(p.nb, p.ncouls, p.ngpown, p.nfreq, p.aqsm, p.aqsn, p.I_eps, p.vcoul, bad, p.indinv)
+    assert e.value.code == fk.FK_ERR_INDEX
+
+
+def test_ff_workspace_size(fk):
+    small = fk.ff_workspace_size(6, 40, 5, 3)
+    big = fk.ff_workspace_size(256, 4096, 4096, 32)
+    assert 0 < small < big < 64 << 20   # cf + column weights + per-tile partials only
