@@ -150,8 +150,9 @@ def default_options() -> Options:
 # --------------------------------------------------------------------------
 def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv_igp_index, indinv, *,
                to1=1e-6, limittwo=0.25, limitone=2.5e5, occ=1.0, precision="fp64", mask=True,
-               stream=None, workspace=None, config=0, time_kernel=False, device=-1):
-    """Run gpp_kernel.  Returns (achtemp, asxtemp, kernel_ms|None).
+               stream=None, workspace=None, config=0, time_kernel=False, device=-1, acht_n1=False):
+    """Run gpp_kernel.  Returns (achtemp, asxtemp, kernel_ms|None), and with
+    acht_n1=True (achtemp, asxtemp, kernel_ms|None, acht_n1_loc[nb]).
 
     numpy inputs -> host memory mode, outputs are numpy complex128 arrays.
     torch device tensors -> device mode, outputs are torch complex128 device
@@ -181,11 +182,14 @@ def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv
         import torch
         ach = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
         asx = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
+        n1 = torch.empty(max(int(nb), 1), dtype=torch.complex128, device=aqsn.device) if acht_n1 else None
     else:
         ach = np.zeros(nw, np.complex128)
         asx = np.zeros(nw, np.complex128)
-    _check(lib.gpp_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(ach), _ptr(asx), None))
-    return ach, asx, (float(ms.value) if time_kernel else None)
+        n1 = np.zeros(max(int(nb), 1), np.complex128) if acht_n1 else None
+    _check(lib.gpp_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(ach), _ptr(asx), _ptr(n1)))
+    out = (ach, asx, (float(ms.value) if time_kernel else None))
+    return out + (n1[:int(nb)],) if acht_n1 else out
 
 
 def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0):

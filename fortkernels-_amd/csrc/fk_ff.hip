@@ -227,8 +227,8 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
 }
 
 // schDt[n1][f], schDv[n1][f]: sum over the column tiles in order
-__global__ void ff_reduce_kernel(const double2 *partials, int nb, int nfreq, int mtiles, double2 *schDt,
-                                 double2 *schDv)
+__global__ void ff_reduce_kernel(const double2 *partials, int nb, int nfreq, int mtiles, double vscale,
+                                 double2 *schDt, double2 *schDv)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb * nfreq) return;
@@ -239,8 +239,8 @@ __global__ void ff_reduce_kernel(const double2 *partials, int nb, int nfreq, int
         st = cadd(st, p[0]);
         sv = cadd(sv, p[1]);
     }
-    schDt[idx] = st;
-    if (schDv) schDv[idx] = sv;
+    if (schDt) schDt[idx] = st;
+    if (schDv) schDv[idx] = make_double2(vscale * sv.x, vscale * sv.y);
 }
 
 // achsDtemp = -0.5 sum_n1 schDv[n1][achs_freq], one workgroup, fixed order
@@ -337,7 +337,7 @@ int ff_check(const fk_ff_problem *p, const fk_options *o)
 
 int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
               const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *X,
-              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, hipStream_t s)
+              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s)
 {
     const int nb = p->number_bands, C = p->ncouls, G = p->ngpown, nf = p->nfreq;
     double2 *cf = static_cast<double2 *>(ws.at(L.off_cf));
@@ -373,8 +373,8 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
     launch_ff_fused(L.cfg, tiles, s, a);
     FK_HIP(hipGetLastError());
     FK_TRY(timer.stop());
-    hipLaunchKernelGGL(ff_reduce_kernel, dim3(div_up(nb * nf, 256)), dim3(256), 0, s, part, nb, nf, L.mtiles, schDt,
-                       schDv);
+    hipLaunchKernelGGL(ff_reduce_kernel, dim3(div_up(nb * nf, 256)), dim3(256), 0, s, part, nb, nf, L.mtiles, vscale,
+                       schDt, schDv);
     FK_HIP(hipGetLastError());
     if (achs) {
         hipLaunchKernelGGL(ff_achs_kernel, dim3(1), dim3(256), 0, s, part, nb, nf, L.mtiles, p->achs_freq, achs);
@@ -384,6 +384,23 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
 }
 
 }  // namespace
+
+int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+                     const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
+                     hipStream_t s)
+{
+    fk_ff_problem p;
+    p.number_bands = nb; p.ncouls = C; p.ngpown = G; p.nfreq = 1;
+    p.aqsmtemp = aqsm; p.aqsntemp = aqsn; p.I_eps_array = X; p.vcoul = vcoul;
+    p.inv_igp_index = nullptr; p.indinv = nullptr; p.achs_freq = 0;
+    fk_options o = fk_default_options();
+    o.memory = FK_MEM_DEVICE;
+    Scratch ws;
+    FfLayout L = ff_plan(&p, &o, ws);
+    FK_TRY(ws.acquire(nullptr, 0, s));
+    return ff_launch(&p, &o, L, ws, cols, aqsm, aqsn, X, vcoul, nullptr, out, nullptr, vscale, s);
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -425,7 +442,7 @@ int ff_kernel(const fk_ff_problem *p, const fk_options *o, fk_cdouble *schDt, fk
         if (rc == FK_OK && o->memory == FK_MEM_DEVICE) {
             rc = ff_launch(p, o, L, ws, cols, (const double2 *)p->aqsmtemp, (const double2 *)p->aqsntemp,
                            (const double2 *)p->I_eps_array, p->vcoul, (double2 *)schDt, (double2 *)schDv,
-                           (double2 *)achsDtemp, s);
+                           (double2 *)achsDtemp, 1.0, s);
         } else if (rc == FK_OK) {
             double2 *dm = nullptr, *dn = nullptr, *dx = nullptr, *dout = nullptr;
             double *dv = nullptr;
@@ -448,7 +465,7 @@ int ff_kernel(const fk_ff_problem *p, const fk_options *o, fk_cdouble *schDt, fk
             up(dx, p->I_eps_array, XC * sizeof(double2));
             up(dv, p->vcoul, sizeof(double) * p->ncouls);
             if (rc == FK_OK)
-                rc = ff_launch(p, o, L, ws, cols, dm, dn, dx, dv, dout, dout + nout, dout + 2 * nout, s);
+                rc = ff_launch(p, o, L, ws, cols, dm, dn, dx, dv, dout, dout + nout, dout + 2 * nout, 1.0, s);
             if (rc == FK_OK) {
                 hipError_t e = hipMemcpyAsync(schDt, dout, sizeof(double2) * nout, hipMemcpyDeviceToHost, s);
                 if (e == hipSuccess && schDv)

@@ -473,6 +473,33 @@ __global__ __launch_bounds__(256) void gpp_reduce_kernel(const double2 *partials
     }
 }
 
+// Band-invariant pole factor of one frequency for the per-band output:
+// X[my_igp][ig] = mask * delw * I_eps at wx (IEEE quotient, the reference's
+// evaluation order), zero where the mask is off.  FP64 from either storage.
+template <class E>
+__global__ void pole_factor_kernel(const E *I_eps, const E *wtilde, int64_t n, double wx, double limittwo,
+                                   double limitone, int mask_on, double2 *X)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double2 wt = widen(wtilde[i]);
+        const double2 ie = widen(I_eps[i]);
+        const double wdr = wx - wt.x;
+        const double wdiffr = pole_wdiffr(wdr, -wt.y);
+        double2 delw;
+        double delwr;
+        pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
+        const bool m = (wdiffr > limittwo) && (delwr < limitone);
+        X[i] = (m || !mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
+    }
+}
+
+__global__ void widen_kernel(const float2 *src, int64_t n, double2 *dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = widen(src[i]);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -635,10 +662,48 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     return timer.finish();
 }
 
+// acht_n1_loc[n1] = sum_my_igp 0.5 vcoul[igp] conj(aqsm[n1][igp]) sum_ig aqsn[n1][ig] X[my_igp][ig]
+// with X the pole factor at frequency index 2 (BGW-Kernels/gppKer.cpp:424):
+// one more complex GEMM over all bands, run as the FF row contraction.
+template <class E>
+int launch_acht_n1(const fk_gpp_problem *p, const fk_options *o, const std::vector<int32_t> &cols, const E *aqsm,
+                   const E *aqsn, const E *ieps, const E *wt, const double *vcoul, double2 *d_n1, hipStream_t s)
+{
+    const int nb = p->number_bands, C = p->ncouls, G = p->ngpown;
+    if (nb == 0) return FK_OK;
+    const bool wide = sizeof(E) == sizeof(double2);
+    Scratch ws;
+    const size_t off_x = ws.plan(sizeof(double2) * (size_t)G * C);
+    const size_t off_m = wide ? 0 : ws.plan(sizeof(double2) * (size_t)nb * C);
+    const size_t off_n = wide ? 0 : ws.plan(sizeof(double2) * (size_t)nb * C);
+    FK_TRY(ws.acquire(nullptr, 0, s));
+    double2 *X = static_cast<double2 *>(ws.at(off_x));
+    const int64_t gc = (int64_t)G * C;
+    const int blocks = (int)std::min<int64_t>((gc + 255) / 256, 8192);
+    hipLaunchKernelGGL(pole_factor_kernel<E>, dim3(blocks), dim3(256), 0, s, ieps, wt, gc, p->wx[2], p->limittwo,
+                       p->limitone, o->mask ? 1 : 0, X);
+    FK_HIP(hipGetLastError());
+    const double2 *am, *an;
+    if constexpr (sizeof(E) == sizeof(double2)) {
+        am = aqsm;
+        an = aqsn;
+    } else {
+        double2 *wm = static_cast<double2 *>(ws.at(off_m)), *wn = static_cast<double2 *>(ws.at(off_n));
+        const int64_t nbc = (int64_t)nb * C;
+        const int b2 = (int)std::min<int64_t>((nbc + 255) / 256, 8192);
+        hipLaunchKernelGGL(widen_kernel, dim3(b2), dim3(256), 0, s, aqsm, nbc, wm);
+        hipLaunchKernelGGL(widen_kernel, dim3(b2), dim3(256), 0, s, aqsn, nbc, wn);
+        FK_HIP(hipGetLastError());
+        am = wm;
+        an = wn;
+    }
+    return ff_rows_weighted(nb, C, G, am, an, X, vcoul, cols, 0.5, d_n1, s);
+}
+
 // Host-memory path: upload, run, download (synchronous).
 template <class E>
 int run_host(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scratch &ws,
-             const std::vector<int32_t> &cols, double2 *ach, double2 *asx, hipStream_t s)
+             const std::vector<int32_t> &cols, double2 *ach, double2 *asx, double2 *n1, hipStream_t s)
 {
     const size_t nbC = (size_t)p->number_bands * p->ncouls, GC = (size_t)p->ngpown * p->ncouls;
     E *dm = nullptr, *dn = nullptr, *die = nullptr, *dwt = nullptr;
@@ -659,16 +724,20 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scra
     alloc((void **)&die, GC * sizeof(E));
     alloc((void **)&dwt, GC * sizeof(E));
     alloc((void **)&dv, sizeof(double) * p->ncouls);
-    alloc((void **)&dout, sizeof(double2) * 2 * p->nfreq);
+    const size_t nout = 2 * (size_t)p->nfreq + (n1 ? (size_t)p->number_bands : 0);
+    alloc((void **)&dout, sizeof(double2) * nout);
     copy(dm, p->aqsmtemp, nbC * sizeof(E));
     copy(dn, p->aqsntemp, nbC * sizeof(E));
     copy(die, p->I_eps_array, GC * sizeof(E));
     copy(dwt, p->wtilde_array, GC * sizeof(E));
     copy(dv, p->vcoul, sizeof(double) * p->ncouls);
     if (rc == FK_OK) rc = launch_gpp<E>(p, o, L, ws, cols, dm, dn, die, dwt, dv, dout, dout + p->nfreq, s);
+    if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, dout + 2 * p->nfreq, s);
     if (rc == FK_OK) {
         hipError_t e = hipMemcpyAsync(ach, dout, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipMemcpyAsync(asx, dout + p->nfreq, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && n1 && p->number_bands > 0)
+            e = hipMemcpyAsync(n1, dout + 2 * p->nfreq, sizeof(double2) * p->number_bands, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { set_error("download: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
     } else {
@@ -702,9 +771,9 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
     clear_error();
     FK_TRY(check_problem(p, o));
     if (!achtemp || !asxtemp) { set_error("gpp_kernel: NULL output"); return FK_ERR_ARG; }
-    if (acht_n1_loc) {
-        set_error("gpp_kernel: acht_n1_loc is not available in this build");
-        return FK_ERR_UNSUPPORTED;
+    if (acht_n1_loc && p->nfreq < 3) {   // the per-band sum is taken at frequency index 2 (:424)
+        set_error("gpp_kernel: acht_n1_loc needs nfreq >= 3 (got %d)", p->nfreq);
+        return FK_ERR_ARG;
     }
     std::vector<int32_t> igp;
     FK_TRY(resolve_columns(p->ncouls, p->ngpown, p->inv_igp_index, p->indinv, igp));
@@ -722,19 +791,23 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
         Layout L = plan_layout(p, o, ws, mixed ? sizeof(float2) : sizeof(double2));
         rc = ws.acquire(o->workspace, o->workspace_bytes, s);
         double2 *ach = reinterpret_cast<double2 *>(achtemp), *asx = reinterpret_cast<double2 *>(asxtemp);
+        double2 *n1 = reinterpret_cast<double2 *>(acht_n1_loc);
         if (rc == FK_OK) {
             if (o->memory == FK_MEM_DEVICE) {
-                if (mixed)
-                    rc = launch_gpp<float2>(p, o, L, ws, igp, (const float2 *)p->aqsmtemp, (const float2 *)p->aqsntemp,
-                                            (const float2 *)p->I_eps_array, (const float2 *)p->wtilde_array, p->vcoul,
-                                            ach, asx, s);
-                else
-                    rc = launch_gpp<double2>(p, o, L, ws, igp, (const double2 *)p->aqsmtemp, (const double2 *)p->aqsntemp,
-                                             (const double2 *)p->I_eps_array, (const double2 *)p->wtilde_array,
-                                             p->vcoul, ach, asx, s);
+                if (mixed) {
+                    const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
+                    const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
+                    rc = launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
+                    if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                } else {
+                    const double2 *m = (const double2 *)p->aqsmtemp, *n = (const double2 *)p->aqsntemp;
+                    const double2 *ie = (const double2 *)p->I_eps_array, *w = (const double2 *)p->wtilde_array;
+                    rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
+                    if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                }
             } else {
-                rc = mixed ? run_host<float2>(p, o, L, ws, igp, ach, asx, s)
-                       : run_host<double2>(p, o, L, ws, igp, ach, asx, s);
+                rc = mixed ? run_host<float2>(p, o, L, ws, igp, ach, asx, n1, s)
+                       : run_host<double2>(p, o, L, ws, igp, ach, asx, n1, s);
             }
         }
     }

@@ -81,4 +81,13 @@ private:
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
+// FP64 row contraction on the device, shared by ff_kernel and the per-band
+// output of gpp_kernel:
+//   out[n1] = vscale * sum_m vcoul[igp_m] conj(aqsm[n1][igp_m]) sum_ig aqsn[n1][ig] X[m][ig]
+// for one X[G][C] (igp_m = cols[m], -1 = skipped column).  Asynchronous on
+// `s`; its scratch is a stream-ordered allocation.
+int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+                     const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
+                     hipStream_t s);
+
 }  // namespace fk

@@ -1,2 +1,213 @@
-// placeholder
-int main() { return 0; }
+// ffKer -- drop-in for the reference's ffKernel program (FFKer_cori/ffKernel.cpp:302-560)
+// for its static-CH sum achsDtemp (:107-131, printed at :528-529) and the FF
+// frequency contraction schDt (ffkernel.deslippe.best.f90.back1:402-418),
+// both from one ff_kernel() call on the MI355X.
+//
+//   ffKer number_bands nvband ncouls ngpown nFreq nfreqeval [options]
+//
+// Without options it builds the reference's inputs (aqsmtemp = aqsntemp =
+// 0.5+0.5i, I_epsR = 0.5+0.5i, I_epsA = 0.5-0.5i, vcoul = 1,
+// inv_igp_index[i] = i, indinv[i] = i; :343-437) in host memory and prints
+// achsDtemp as the reference does, "( %f, %f) " (Complex.h:91-94), then at
+// full precision.  A wrong argument count, or ngpown > ncouls, prints the
+// reference's message and exits with status 0 (:305-325).
+//
+// The C++ program reads frequency index 1 for achsDtemp (:122); the Fortran
+// original reads the first frequency (ffkernel.deslippe.best.f90:263).  The
+// default follows the program this replaces; --achs-freq 0 gives the Fortran.
+//
+// Options:
+//   --random SEED    seeded inputs generated on the device: aqs*, I_epsR,
+//                    I_epsA ~ U(-0.5,0.5) + iU(-0.5,0.5), vcoul U(0.1,10)
+//   --achs-freq F    frequency index of achsDtemp (default 1)
+//   --both           also contract I_epsA (the Fortran contracts both)
+//   --repeat K       run K times, print the mean kernel time
+//   --device D       HIP device ordinal
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "fortkernels.h"
+
+namespace {
+
+void usage_params()
+{
+    std::cout << "./a.out number_bands nvband ncouls ngpown nFreq nfreqeval "
+                 "[--random SEED] [--achs-freq F] [--both] [--repeat K] [--device D]"
+              << std::endl;
+}
+
+[[noreturn]] void fail(const char *what, int rc)
+{
+    std::cerr << "ffKer: " << what << " failed (" << rc << "): " << fk_last_error() << std::endl;
+    std::exit(1);
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        std::cerr << "ffKer: " << what << ": " << hipGetErrorString(e) << std::endl;
+        std::exit(1);
+    }
+}
+
+struct Args {
+    int nb = 0, nv = 0, ncouls = 0, ngpown = 0, nfreq = 0, nfreqeval = 0;
+    bool random = false, both = false;
+    unsigned long long seed = 0;
+    int achs_freq = 1, repeat = 1, device = -1;
+};
+
+bool parse(int argc, char **argv, Args &a)
+{
+    std::vector<std::string> pos;
+    for (int i = 1; i < argc; ++i) {
+        std::string s = argv[i];
+        auto next = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : nullptr; };
+        if (s.rfind("--", 0) != 0) { pos.push_back(s); continue; }
+        const char *v = nullptr;
+        if (s == "--random") { if (!(v = next())) return false; a.random = true; a.seed = std::strtoull(v, nullptr, 10); }
+        else if (s == "--achs-freq") { if (!(v = next())) return false; a.achs_freq = std::atoi(v); }
+        else if (s == "--both") a.both = true;
+        else if (s == "--repeat") { if (!(v = next())) return false; a.repeat = std::atoi(v); }
+        else if (s == "--device") { if (!(v = next())) return false; a.device = std::atoi(v); }
+        else return false;
+    }
+    if (pos.size() != 6) return false;
+    int *dst[] = {&a.nb, &a.nv, &a.ncouls, &a.ngpown, &a.nfreq, &a.nfreqeval};
+    for (int k = 0; k < 6; ++k) *dst[k] = std::atoi(pos[k].c_str());
+    return a.repeat >= 1;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    Args a;
+    if (!parse(argc, argv, a)) {
+        std::cout << "Incorrect Parameters!!! The correct form is " << std::endl;
+        usage_params();
+        return 0;
+    }
+    if (a.ngpown > a.ncouls) {
+        std::cout << "Incorrect Parameters!!! ngpown cannot be greater than ncouls. The correct form is " << std::endl;
+        usage_params();
+        return 0;
+    }
+    const int nb = a.nb, C = a.ncouls, G = a.ngpown, nf = a.nfreq;
+    if (nb < 1 || C < 1 || G < 1 || nf < 1 || a.achs_freq < 0 || a.achs_freq >= nf) {
+        std::cerr << "ffKer: need number_bands, ncouls, ngpown, nFreq >= 1 and 0 <= achs-freq < nFreq" << std::endl;
+        return 2;
+    }
+    const auto t_start = std::chrono::high_resolution_clock::now();
+    if (a.device >= 0) hip_check(hipSetDevice(a.device), "hipSetDevice");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    std::cout << "Device = " << prop.name << " (" << fk_build_target() << ", " << prop.multiProcessorCount
+              << " CUs)" << "\n number_bands = " << nb << "\n nvband = " << a.nv << "\n ncouls = " << C
+              << "\n ngpown = " << G << "\n nFreq = " << nf << "\n nfreqeval = " << a.nfreqeval << std::endl;
+
+    std::vector<int32_t> inv(G), indinv(C + 1);
+    for (int i = 0; i < G; ++i) inv[i] = i;
+    for (int i = 0; i < C; ++i) indinv[i] = i;
+    indinv[C] = C - 1;
+    const size_t nbc = (size_t)nb * C, xc = (size_t)nf * G * C;
+    const int narr = a.both ? 2 : 1;
+    std::cout << "Memory Used = " << (double)(2 * nbc + narr * xc) * sizeof(fk_cdouble) / (1024.0 * 1024 * 1024)
+              << " GB" << std::endl;
+
+    fk_options opt = fk_default_options();
+    opt.device = a.device;
+    std::vector<fk_cdouble> h_aqs, h_x[2];
+    std::vector<double> h_v;
+    void *d_m = nullptr, *d_n = nullptr, *d_x[2] = {nullptr, nullptr};
+    double *d_v = nullptr;
+    const void *pm, *pn, *px[2] = {nullptr, nullptr};
+    const double *pv;
+    if (!a.random) {
+        opt.memory = FK_MEM_HOST;
+        h_aqs.assign(nbc, fk_cdouble{0.5, 0.5});
+        h_x[0].assign(xc, fk_cdouble{0.5, 0.5});       // I_epsR
+        if (a.both) h_x[1].assign(xc, fk_cdouble{0.5, -0.5});   // I_epsA
+        h_v.assign(C, 1.0);
+        pm = pn = h_aqs.data();
+        px[0] = h_x[0].data();
+        px[1] = a.both ? h_x[1].data() : nullptr;
+        pv = h_v.data();
+    } else {
+        opt.memory = FK_MEM_DEVICE;
+        hip_check(hipMalloc(&d_m, nbc * sizeof(fk_cdouble)), "hipMalloc aqsmtemp");
+        hip_check(hipMalloc(&d_n, nbc * sizeof(fk_cdouble)), "hipMalloc aqsntemp");
+        for (int k = 0; k < narr; ++k) hip_check(hipMalloc(&d_x[k], xc * sizeof(fk_cdouble)), "hipMalloc I_eps");
+        hip_check(hipMalloc((void **)&d_v, C * sizeof(double)), "hipMalloc vcoul");
+        int rc = fk_fill_complex(d_m, (int64_t)nbc, a.seed, 1, -0.5, 0.5, -0.5, 0.5, 0, nullptr);
+        if (rc == FK_OK) rc = fk_fill_complex(d_n, (int64_t)nbc, a.seed, 2, -0.5, 0.5, -0.5, 0.5, 0, nullptr);
+        for (int k = 0; k < narr && rc == FK_OK; ++k)
+            rc = fk_fill_complex(d_x[k], (int64_t)xc, a.seed, 6 + k, -0.5, 0.5, -0.5, 0.5, 0, nullptr);
+        if (rc == FK_OK) rc = fk_fill_real(d_v, C, a.seed, 5, 0.1, 10.0, nullptr);
+        if (rc != FK_OK) fail("input fill", rc);
+        hip_check(hipDeviceSynchronize(), "input fill");
+        pm = d_m; pn = d_n; px[0] = d_x[0]; px[1] = d_x[1]; pv = d_v;
+    }
+    const std::chrono::duration<double> preloop = std::chrono::high_resolution_clock::now() - t_start;
+    std::cout << "********** PreLoop **********= " << preloop.count() << " secs" << std::endl;
+    std::cout << "starting Kernels" << std::endl;
+
+    const size_t nout = (size_t)nb * nf;
+    std::vector<fk_cdouble> schDt[2], achs(2);
+    fk_cdouble *d_out = nullptr;
+    if (a.random) hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * 2 * (nout + 1)), "hipMalloc outputs");
+    float kms = 0.f;
+    double kms_sum[2] = {0.0, 0.0};
+    opt.kernel_ms = &kms;
+    const auto t_k = std::chrono::high_resolution_clock::now();
+    for (int k = 0; k < narr; ++k) {
+        schDt[k].resize(nout);
+        fk_ff_problem p;
+        std::memset(&p, 0, sizeof(p));
+        p.number_bands = nb; p.ncouls = C; p.ngpown = G; p.nfreq = nf;
+        p.aqsmtemp = pm; p.aqsntemp = pn; p.I_eps_array = px[k]; p.vcoul = pv;
+        p.inv_igp_index = inv.data(); p.indinv = indinv.data();
+        p.achs_freq = a.achs_freq;
+        fk_cdouble *o_t = a.random ? d_out + k * (nout + 1) : schDt[k].data();
+        fk_cdouble *o_a = a.random ? d_out + k * (nout + 1) + nout : &achs[k];
+        for (int r = 0; r < a.repeat; ++r) {
+            const int rc = ff_kernel(&p, &opt, o_t, nullptr, o_a);
+            if (rc != FK_OK) fail("ff_kernel", rc);
+            kms_sum[k] += kms;
+        }
+        if (a.random) {
+            hip_check(hipMemcpy(schDt[k].data(), o_t, sizeof(fk_cdouble) * nout, hipMemcpyDeviceToHost), "copy");
+            hip_check(hipMemcpy(&achs[k], o_a, sizeof(fk_cdouble), hipMemcpyDeviceToHost), "copy");
+        }
+    }
+    const std::chrono::duration<double> kernel_time = std::chrono::high_resolution_clock::now() - t_k;
+
+    std::printf("achsDtemp = ( %f, %f) \n", achs[0].re, achs[0].im);
+    std::printf("achsDtemp (full precision) = (%.15e, %.15e)\n", achs[0].re, achs[0].im);
+    const char *names[2] = {"I_epsR", "I_epsA"};
+    const double flops = 8.0 * (double)G * nf * nb * C;
+    for (int k = 0; k < narr; ++k) {
+        double sr = 0, si = 0;
+        for (size_t i = 0; i < nout; ++i) { sr += schDt[k][i].re; si += schDt[k][i].im; }
+        std::printf("schDt[%s]: sum over %d bands x %d frequencies = (%.15e, %.15e); schDt[0][0] = (%.15e, %.15e)\n",
+                    names[k], nb, nf, sr, si, schDt[k][0].re, schDt[k][0].im);
+        const double ms = kms_sum[k] / a.repeat;
+        std::printf("Kernel time [%s] = %.4f ms per call (%.2f TFLOP/s FP64, %d call%s)\n", names[k], ms,
+                    flops / (ms * 1e-3) / 1e12, a.repeat, a.repeat > 1 ? "s" : "");
+    }
+    std::cout << "********** Total Kernel Time **********= " << kernel_time.count() << " secs" << std::endl;
+
+    for (void *ptr : {d_m, d_n, d_x[0], d_x[1], (void *)d_v, (void *)d_out})
+        if (ptr) (void)hipFree(ptr);
+    return 0;
+}
