@@ -12,6 +12,11 @@ over its HIP-event-timed duration against the 78.6 TFLOP/s FP64 MFMA peak.
 cpu_baseline = the reference's own gppKer program (oracle/_ref, built from
 the reference sources) on a bounded band sample of the same shape, timed on
 this host's cores; the CPU oracle restatement is used if it is absent.
+
+Two more objects ride on the same line: "mixed" (the complex64-storage GPP
+form on the same shape) and "ff" (the FF frequency contraction at
+number_bands 256, ncouls = ngpown = 4096, nFreq 32, FP64 MFMA, with its own
+roofline against the FP64 MFMA peak).
 """
 import argparse
 import json
@@ -128,6 +133,57 @@ def run_gpp(fk, torch, precision, steps, warmup, seed=1):
                 kernel_ms_min=float(np.min(kms)), achtemp=ach, C=C, G=G)
 
 
+FF_NB, FF_C, FF_G, FF_NF = 256, 4096, 4096, 32
+
+
+def run_ff(fk, torch, steps, warmup, seed=4):
+    """ff_kernel on I_epsR at 256/4096/4096/32, inputs resident in HBM."""
+    dev = torch.device("cuda:0")
+    nb, C, G, nf = FF_NB, FF_C, FF_G, FF_NF
+    log(f"ff: generating inputs nb={nb} ncouls={C} ngpown={G} nfreq={nf} (I_epsR {16 * nf * G * C / 1e9:.2f} GB)")
+    aqsm = torch.empty(nb, C, dtype=torch.complex128, device=dev)
+    aqsn = torch.empty(nb, C, dtype=torch.complex128, device=dev)
+    I = torch.empty(nf, G, C, dtype=torch.complex128, device=dev)
+    v = torch.empty(C, dtype=torch.float64, device=dev)
+    fk.fill_complex(aqsm, seed, 1, (-0.5, 0.5), (-0.5, 0.5))
+    fk.fill_complex(aqsn, seed, 2, (-0.5, 0.5), (-0.5, 0.5))
+    fk.fill_complex(I, seed, 6, (-0.5, 0.5), (-0.5, 0.5))
+    fk.fill_real(v, seed, 5, 0.1, 10.0)
+    inv = np.arange(G, dtype=np.int32)
+    indinv = np.arange(C + 1, dtype=np.int32)
+    indinv[C] = C - 1
+    nbytes = fk.ff_workspace_size(nb, C, G, nf)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step(time_kernel=False):
+        return fk.ff_kernel(nb, C, G, nf, aqsm, aqsn, I, v, inv, indinv, stream=stream, workspace=ws,
+                            time_kernel=time_kernel)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"ff: timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [step(time_kernel=True)[3] for _ in range(max(steps, 3))]
+    flops = 8.0 * G * nf * nb * C
+    kernel_ms = float(np.mean(kms))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    ms = 1e3 * wall / steps
+    log(f"ff: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {achieved:.1f} TFLOP/s "
+        f"({100 * achieved / PEAK_FP64:.1f}% of FP64 peak)")
+    return {"workload": f"FF contraction {nb}/{C}/{G}/{nf} (number_bands/ncouls/ngpown/nFreq), FP64, I_epsR",
+            "value": nb * G * C * nf / (ms * 1e-3), "unit": "complex MACs/s", "ms_per_step": ms,
+            "roofline": {"bound": "mfma", "kernel": "ff_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
+                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                         "traffic": load_traffic("ff_fused_fp64_traffic.json"),
+                         "alg_flops_per_launch": flops, "kernel_ms": kernel_ms}}
+
+
 def cpu_baseline(bands, threads):
     """The reference's own gppKer (oracle/_ref) on `bands` bands of the
     512/2/32768/20 shape, OMP threads = `threads`."""
@@ -171,6 +227,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ff", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -222,6 +279,8 @@ def main():
                          "achieved_tflops": ach_m, "peak_fp32_tflops": PEAK_FP32,
                          "workload": "same shape, complex64 storage, wtilde re ~ U(3,5)"}
         log(f"mixed: {m['ms_per_step']:.3f} ms/step ({line['mixed']['speedup_vs_fp64']:.2f}x vs fp64)")
+    if not args.no_ff:
+        line["ff"] = run_ff(fk, torch, max(3, args.steps // 4), 1)
     if args.calibrate_copy:
         n = args.calibrate_copy << 20
         src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
