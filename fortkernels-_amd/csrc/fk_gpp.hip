@@ -24,6 +24,7 @@
 //   gpp_reduce_kernel fixed-order sum of the per-tile partials -> outputs
 // Every sum is taken in a fixed order, so results are bitwise reproducible.
 #include "fk_complex.h"
+#include "fk_gpp_common.h"
 #include "fk_internal.h"
 
 #include <algorithm>
@@ -32,17 +33,10 @@
 namespace fk {
 namespace {
 
+using namespace gpp;
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-
-constexpr int kMaxFreq = 16;
-constexpr int kXcds = 8;       // MI355X: consecutive workgroups go to different XCDs
-
-struct PoleConst {
-    double wx[kMaxFreq];
-    double wx2[kMaxFreq];      // cden = pow(wxt,2) of the mask branch (:41)
-    double rden1[kMaxFreq];    // 1 / |cden|^2 (:42-43)
-};
 
 template <class E>
 struct GppArgs {
@@ -58,9 +52,7 @@ struct GppArgs {
     int tiles_m, tiles_n;
     int stagger_lo, stagger_hi;   // workgroups that start late ...
     int stagger_ticks;            // ... by this many wall-clock ticks (100 MHz)
-    double limittwo, limitone, to1;
-    int mask_on;
-    PoleConst pc;
+    PoleParams pp;
 };
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
@@ -87,9 +79,6 @@ template <class E> struct Acc;
 template <> struct Acc<double2> { using type = d4; };
 template <> struct Acc<float2> { using type = f4; };
 
-__device__ __forceinline__ double2 widen(double2 v) { return v; }
-__device__ __forceinline__ double2 widen(float2 v) { return make_double2((double)v.x, (double)v.y); }
-
 // Row of the 16x16 MFMA result held in accumulator register r of lane l
 // (column l % 16): FP64 16x16x4 interleaves rows across registers, FP32
 // 16x16x4 keeps four consecutive rows per lane group.
@@ -111,50 +100,6 @@ __device__ __forceinline__ void tile_coords(int b, int tiles_m, int total, int &
     tn = t / tiles_m;
 }
 
-// 1/x from v_rcp_f64 refined by two Newton steps (within an ulp or two of
-// the IEEE quotient).
-__device__ __forceinline__ double rcp_fast(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-// delw = wtilde * conj(wdiff) * rden and |delw|^2 in the reference's FP64
-// evaluation order (BGW-Kernels/gppKer.cpp:30-36, :396-402).  Contraction is
-// off so, given the same rden, they round exactly as the reference's.
-__device__ __forceinline__ void pole_delw(double2 wt, double wdr, double rden, double2 &delw, double &delwr)
-{
-#pragma clang fp contract(off)
-    const double wdi = -wt.y;
-    delw = make_double2((wt.x * wdr - wt.y * (-wdi)) * rden, (wt.x * (-wdi) + wt.y * wdr) * rden);
-    delwr = delw.x * delw.x + delw.y * delw.y;
-}
-
-__device__ __forceinline__ double pole_wdiffr(double wdr, double wdi)
-{
-#pragma clang fp contract(off)
-    return wdr * wdr + wdi * wdi;
-}
-
-// ssx of the near-pole branch (:46-53): -Omega2 conj(cden)/|cden|^2 delw,
-// cden = 4 wtilde^2 (delw + 0.5).
-__device__ __forceinline__ double2 ssx_branch2(double2 wt2, double2 Om, double2 delw)
-{
-    double2 cden = cmul(make_double2(4.0 * wt2.x, 4.0 * wt2.y), make_double2(delw.x + 0.5, delw.y));
-    double rd = 1.0 / (cden.x * cden.x + cden.y * cden.y);
-    double2 t = cmul(make_double2(-Om.x, -Om.y), cconj(cden));
-    return cmul(make_double2(t.x * rd, t.y * rd), delw);
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // c[kr][m] = 0.5 vcoul[igp] conj(aqsm[band][igp]) for a kept column m < G,
@@ -350,7 +295,6 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
         }
 
     const size_t tile = (size_t)tm + (size_t)tn * a.tiles_m;
-    const double tol1 = 1e-13 * a.limitone, tol2 = 1e-13 * a.to1;
     for (int w0 = 0; w0 < a.nw; w0 += NWC) {
         double2 sach[NWC], sasx[NWC];
 #pragma unroll
@@ -371,43 +315,9 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const bool ok = ((rowok >> (i * 4 + r)) & 1u) && col <= nmax;
-                    const double2 wt = widen(cwt[r]);
-                    const double2 ie = widen(cie[r]);
                     const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
                     const double2 Bo = widen(occ[((i * NB + j) * 4 + r) * 64 + lane]);
-                    const double2 wt2 = cmul(wt, wt);
-                    const double2 Om = cmul(wt2, ie);
-#pragma unroll
-                    for (int q = 0; q < NWC; ++q) {
-                        const int iw = min(w0 + q, a.nw - 1);   // a short last pass repeats a frequency ...
-                        const bool live = ok && (w0 + q < a.nw); // ... and adds nothing for it
-                        const double wdr = a.pc.wx[iw] - wt.x;
-                        const double wdiffr = pole_wdiffr(wdr, -wt.y);
-                        double2 delw;
-                        double delwr;
-                        pole_delw(wt, wdr, rcp_fast(wdiffr), delw, delwr);
-                        // points within 1e-13 of a branch threshold take the IEEE
-                        // quotient, so every branch decision is the reference's
-                        const bool near = fabs(delwr - a.limitone) <= tol1 || fabs(delwr - a.to1) <= tol2;
-                        if (__ballot(near)) {
-                            if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
-                        }
-                        const bool m = (wdiffr > a.limittwo) && (delwr < a.limitone);
-                        const double2 D = (m || !a.mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
-                        const double2 t = cmul(B, D);
-                        if (live) sach[q] = cadd(sach[q], t);
-                        if (want_ssx) {
-                            // mask branch (:41-44): Omega2 * conj(wx^2) / |wx^2|^2
-                            double2 S = m ? make_double2(Om.x * a.pc.wx2[iw] * a.pc.rden1[iw], Om.y * a.pc.wx2[iw] * a.pc.rden1[iw])
-                                          : make_double2(0.0, 0.0);
-                            const bool need2 = !m && delwr > a.to1;
-                            if (__ballot(need2)) {
-                                if (need2) S = ssx_branch2(wt2, Om, delw);
-                            }
-                            const double2 u = cmul(Bo, S);
-                            if (live) sasx[q] = cadd(sasx[q], u);
-                        }
-                    }
+                    pole_element<NWC>(a.pp, w0, ok, widen(cwt[r]), widen(cie[r]), B, Bo, want_ssx, sach, sasx);
                 }
             }
         // wave reduction, then a fixed-order sum over the waves
@@ -643,14 +553,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
             a.stagger_ticks = (int)std::min(0.5 * tile_cycles / 24.0, 1.0e6);
         }
     }
-    a.limittwo = p->limittwo; a.limitone = p->limitone; a.to1 = p->to1; a.mask_on = o->mask ? 1 : 0;
-    for (int iw = 0; iw < kMaxFreq; ++iw) {
-        double w = iw < p->nfreq ? p->wx[iw] : 0.0;
-        double w2 = w * w;
-        a.pc.wx[iw] = w;
-        a.pc.wx2[iw] = w2;
-        a.pc.rden1[iw] = 1.0 / (w2 * w2);
-    }
+    a.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
     KernelTimer timer(o->kernel_ms, s);
     FK_TRY(timer.start());
     launch_fused<E>(L.cfg, tiles, s, a);
@@ -701,9 +604,9 @@ int launch_acht_n1(const fk_gpp_problem *p, const fk_options *o, const std::vect
 }
 
 // Host-memory path: upload, run, download (synchronous).
-template <class E>
-int run_host(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scratch &ws,
-             const std::vector<int32_t> &cols, double2 *ach, double2 *asx, double2 *n1, hipStream_t s)
+template <class E, class Launch>
+int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int32_t> &cols, double2 *ach,
+             double2 *asx, double2 *n1, hipStream_t s, Launch &&launch)
 {
     const size_t nbC = (size_t)p->number_bands * p->ncouls, GC = (size_t)p->ngpown * p->ncouls;
     E *dm = nullptr, *dn = nullptr, *die = nullptr, *dwt = nullptr;
@@ -731,7 +634,7 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scra
     copy(die, p->I_eps_array, GC * sizeof(E));
     copy(dwt, p->wtilde_array, GC * sizeof(E));
     copy(dv, p->vcoul, sizeof(double) * p->ncouls);
-    if (rc == FK_OK) rc = launch_gpp<E>(p, o, L, ws, cols, dm, dn, die, dwt, dv, dout, dout + p->nfreq, s);
+    if (rc == FK_OK) rc = launch(dm, dn, die, dwt, dv, dout, dout + p->nfreq);
     if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, dout + 2 * p->nfreq, s);
     if (rc == FK_OK) {
         hipError_t e = hipMemcpyAsync(ach, dout, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
@@ -749,6 +652,16 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scra
 }
 
 }  // namespace
+
+int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
+                      double2 *asxtemp, hipStream_t s)
+{
+    hipLaunchKernelGGL(gpp_reduce_kernel, dim3(2 * nw), dim3(256), 0, s, partials, tiles, nw, asx_scale, achtemp,
+                       asxtemp);
+    FK_HIP(hipGetLastError());
+    return FK_OK;
+}
+
 }  // namespace fk
 
 using namespace fk;
@@ -761,7 +674,13 @@ size_t gpp_kernel_workspace_size(const fk_gpp_problem *p, const fk_options *o)
     if (!o) o = &d;
     if (check_problem(p, o) != FK_OK) return 0;
     Scratch ws;
-    plan_layout(p, o, ws, o->precision == FK_PREC_MIXED ? sizeof(float2) : sizeof(double2));
+    if (o->precision == FK_PREC_MIXED && o->config == 0) {
+        OzLayout Z;
+        gpp_oz_plan(p, ws, Z);
+        ws.plan(sizeof(int) * (size_t)p->ngpown);
+    } else {
+        plan_layout(p, o, ws, o->precision == FK_PREC_MIXED ? sizeof(float2) : sizeof(double2));
+    }
     return ws.total();
 }
 
@@ -785,13 +704,47 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
     }
     hipStream_t s = static_cast<hipStream_t>(o->stream);
     const bool mixed = o->precision == FK_PREC_MIXED;
+    // FK_PREC_MIXED with config 0: the int8-split contraction; configs 1..4
+    // select the FP32 MFMA contraction with that tile configuration
+    const bool oz = mixed && o->config == 0;
+    if (oz) FK_TRY(gpp_oz_check(p));
+    double2 *ach = reinterpret_cast<double2 *>(achtemp), *asx = reinterpret_cast<double2 *>(asxtemp);
+    double2 *n1 = reinterpret_cast<double2 *>(acht_n1_loc);
     int rc;
-    {
+    if (oz) {
+        Scratch ws;
+        OzLayout Z;
+        gpp_oz_plan(p, ws, Z);
+        const size_t off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
+        rc = ws.acquire(o->workspace, o->workspace_bytes, s);
+        int *d_igp = static_cast<int *>(ws.at(off_igp));
+        if (rc == FK_OK) {
+            hipError_t e = hipMemcpyAsync(d_igp, igp.data(), sizeof(int) * igp.size(), hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) { set_error("column map upload: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
+        }
+        auto launch = [&](const float2 *m, const float2 *n, const float2 *ie, const float2 *w, const double *v,
+                          double2 *da, double2 *dx) -> int {
+            if (p->number_bands == 0) {
+                hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, da, dx, p->nfreq);
+                FK_HIP(hipGetLastError());
+                return FK_OK;
+            }
+            return gpp_oz_launch(p, o, Z, ws, d_igp, m, n, ie, w, v, da, dx, s);
+        };
+        if (rc == FK_OK) {
+            if (o->memory == FK_MEM_DEVICE) {
+                const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
+                const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
+                rc = launch(m, n, ie, w, p->vcoul, ach, asx);
+                if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+            } else {
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, s, launch);
+            }
+        }
+    } else {
         Scratch ws;
         Layout L = plan_layout(p, o, ws, mixed ? sizeof(float2) : sizeof(double2));
         rc = ws.acquire(o->workspace, o->workspace_bytes, s);
-        double2 *ach = reinterpret_cast<double2 *>(achtemp), *asx = reinterpret_cast<double2 *>(asxtemp);
-        double2 *n1 = reinterpret_cast<double2 *>(acht_n1_loc);
         if (rc == FK_OK) {
             if (o->memory == FK_MEM_DEVICE) {
                 if (mixed) {
@@ -805,9 +758,18 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
                     rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
                     if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
                 }
+            } else if (mixed) {
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, s,
+                                      [&](const float2 *m, const float2 *n, const float2 *ie, const float2 *w,
+                                          const double *v, double2 *da, double2 *dx) {
+                                          return launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);
+                                      });
             } else {
-                rc = mixed ? run_host<float2>(p, o, L, ws, igp, ach, asx, n1, s)
-                       : run_host<double2>(p, o, L, ws, igp, ach, asx, n1, s);
+                rc = run_host<double2>(p, o, igp, ach, asx, n1, s,
+                                       [&](const double2 *m, const double2 *n, const double2 *ie, const double2 *w,
+                                           const double *v, double2 *da, double2 *dx) {
+                                           return launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);
+                                       });
             }
         }
     }

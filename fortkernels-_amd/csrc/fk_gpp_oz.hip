@@ -1,0 +1,541 @@
+// fk_gpp_oz.hip -- mixed-precision GPP band sum on MI355X (gfx950): the
+// band contraction B = sum_n1 c[n1] aqsn[n1] runs on int8 MFMA with exact
+// int32 accumulation (an Ozaki-style split, Ozaki, Ogita, Oishi & Rump,
+// "Error-free transformations of matrix multiplication", Numer. Algorithms
+// 59 (2012); on integer matrix units: Ootomo, Ozaki & Yokota, IJHPCA 2024);
+// everything else is FP64.
+//
+// Inputs are complex64 (fk_cfloat).  Each row of the coefficient matrix
+// c[n1][my_igp] = 0.5 vcoul[igp] conj(aqsm[n1][igp]) (FP64) and each column
+// of aqsn is scaled so that its largest |re|, |im| (M) maps to 127, and
+// split into four int8 slices:
+//     x = M / 127 * (s0 + s1/254 + s2/254^2 + s3/254^3),  |s_k| <= 127,
+// 31 bits of the row's or column's range.  A complex product is a real GEMM
+// over an interleaved K' = 2 nb axis:
+//     re B = sum (cr, -ci) . (ar, ai),   im B = sum (ci, cr) . (ar, ai),
+// and the slice pairs (i, j) with i + j = L share the weight 254^-L, so five
+// int32 accumulators per component (L = 0..4; the pairs with i + j > 4 are
+// below 254^-5 of the leading term and dropped) hold the contraction
+// exactly.  The error against the FP64 evaluation on the same complex64
+// values is the slice truncation's, a few 1e-10 relative.
+//
+// Kernels (one call = six launches on the caller's stream):
+//   oz_max_kernel        partial row / column maxima over 64-band chunks
+//   oz_split_a_kernel    coefficients -> int8 slices, FP64 occupied rows
+//   oz_split_b_kernel    aqsn -> int8 slices
+//   gpp_oz_gemm_kernel   int8 MFMA tile, operands staged through LDS by
+//                        direct global->LDS loads in a three-deep ring;
+//                        B rebuilt in FP64 and stored
+//   gpp_oz_pole_kernel   per (my_igp, ig): occupied-band sum Bo in FP64, the
+//                        pole and SX factors in FP64, per-block partials
+//   gpp_reduce_kernel    fixed-order sum of the partials (shared with FP64)
+//
+// Slices are stored in MFMA-lane order: a 1 KB block per (slice, 32 rows,
+// 32 K') in which lane L's 16 bytes (row L%32, K' 16 (L/32) .. +15) sit at
+// byte 16 L, so one direct-to-LDS load moves a whole operand fragment of
+// the 32x32x32 int8 MFMA and the MFMA reads it back with one conflict-free
+// ds_read_b128 per lane.  (The 16x16x64 int8 MFMA runs at half the int8
+// rate on this part; the 32x32x32 form reaches it.)
+#include "fk_complex.h"
+#include "fk_gpp_common.h"
+#include "fk_internal.h"
+
+#include <algorithm>
+
+namespace fk {
+namespace {
+
+using namespace gpp;
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSlices = 4;
+constexpr int kLevels = 5;      // slice pairs with i + j < kLevels
+constexpr int kKStep = 64;      // K' per pipeline step = 32 bands = two MFMA k-halves
+constexpr int kKHalf = 32;      // K' per 32x32x32 int8 MFMA
+constexpr int kBandsPerStep = kKStep / 2;
+constexpr int kMaxChunk = 64;   // bands per partial maximum
+constexpr int kPlanesA = 2 * kSlices, kPlanesB = kSlices;
+constexpr int kFrag = 1024;     // bytes per lane-ordered fragment
+constexpr int kMaxBandsOz = 16000;   // keeps the level-3 int32 sums (4 pairs) below 2^31
+
+// Slices of x, |x| <= M (scale = 127 / M), in units of M/127 (first) and
+// /254 per level.  The scaling rounds once at 2^-53 relative; every later
+// step is exact.
+__device__ __forceinline__ void split4(double x, double scale, int8_t (&s)[kSlices])
+{
+    double t = x * scale;
+    double q = rint(t);
+    s[0] = (int8_t)q;
+#pragma unroll
+    for (int k = 1; k < kSlices; ++k) {
+        t = (t - q) * 254.0;
+        q = rint(t);
+        s[k] = (int8_t)q;
+    }
+}
+
+
+__device__ __forceinline__ double2 coef(const float2 *aqsm, const double *vcoul, int C, int n1, int g)
+{
+    const float2 am = aqsm[(size_t)n1 * C + g];
+    const double s = 0.5 * vcoul[g];
+    return make_double2((double)am.x * s, -(double)am.y * s);
+}
+
+// ---------------------------------------------------------------------------
+// Partial maxima of max(|re|, |im|) over 64-band chunks (blockIdx.y): rows
+// of the coefficients (the first ldm/64 workgroups of x) and columns of
+// aqsn (the rest).  pmax_a[chunk][ldm], pmax_b[chunk][ldn].
+// ---------------------------------------------------------------------------
+struct MaxArgs {
+    const float2 *aqsm, *aqsn;
+    const double *vcoul;
+    const int *igp;
+    double *pmax_a, *pmax_b;
+    int nb, C, G, ldm, ldn, chunks, blocks_a;
+};
+
+__global__ __launch_bounds__(256) void oz_max_kernel(MaxArgs a)
+{
+    __shared__ double sm[4][64];
+    const int tid = threadIdx.x, cl = tid & 63, grp = tid >> 6;
+    const int chunk = blockIdx.y;
+    const int n_lo = chunk * kMaxChunk, n_hi = min(n_lo + kMaxChunk, a.nb);
+    double mx = 0.0;
+    const bool is_a = (int)blockIdx.x < a.blocks_a;
+    const int idx = (is_a ? (int)blockIdx.x : (int)blockIdx.x - a.blocks_a) * 64 + cl;
+    if (is_a) {
+        const int g = idx < a.G ? a.igp[idx] : -1;
+        if (g >= 0)
+            for (int n1 = n_lo + grp; n1 < n_hi; n1 += 4) {
+                const double2 c = coef(a.aqsm, a.vcoul, a.C, n1, g);
+                mx = fmax(mx, fmax(fabs(c.x), fabs(c.y)));
+            }
+    } else if (idx < a.C) {
+        for (int n1 = n_lo + grp; n1 < n_hi; n1 += 4) {
+            const float2 v = a.aqsn[(size_t)n1 * a.C + idx];
+            mx = fmax(mx, (double)fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+    }
+    sm[grp][cl] = mx;
+    __syncthreads();
+    if (grp == 0) {
+        mx = fmax(fmax(sm[0][cl], sm[1][cl]), fmax(sm[2][cl], sm[3][cl]));
+        if (is_a) a.pmax_a[(size_t)chunk * a.ldm + idx] = mx;
+        else a.pmax_b[(size_t)chunk * a.ldn + idx] = mx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient slices.  Workgroup (x, y) = 64 rows x one k-step (32 bands);
+// thread t: row m = 64 x + t%64, bands 32 y + 8 q .. +7 with q = t/64, i.e.
+// k-half 2 y + q/2 and lane L = m%32 + 32 (q%2) of the fragment
+// (plane, m/32, k-half) of every plane.  Plane 2 s + v holds slice s of
+// the pairs (cr, -ci) for v = 0 and (ci, cr) for v = 1.
+// ---------------------------------------------------------------------------
+struct SplitAArgs {
+    const float2 *aqsm;
+    const double *vcoul;
+    const int *igp;
+    const double *pmax_a;
+    int8_t *as;              // [8][ldm/32][Kp/32][1024]
+    double *ra;              // [ldm] M / 127 of the row
+    double2 *c_occ;          // [nv][ldm]
+    int nb, nv, C, G, ldm, nkh, chunks;
+};
+
+__global__ __launch_bounds__(256) void oz_split_a_kernel(SplitAArgs a)
+{
+    const int tid = threadIdx.x, q = tid >> 6;
+    const int m = blockIdx.x * 64 + (tid & 63);
+    const int ks = blockIdx.y;
+    const int g = m < a.G ? a.igp[m] : -1;
+    double mx = 0.0;
+    for (int c = 0; c < a.chunks; ++c) mx = fmax(mx, a.pmax_a[(size_t)c * a.ldm + m]);
+    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
+    if (ks == 0 && q == 0) a.ra[m] = mx / 127.0;
+
+    int8_t p[kPlanesA][16];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int n1 = ks * kBandsPerStep + 8 * q + b;
+        const double2 c = (g >= 0 && n1 < a.nb) ? coef(a.aqsm, a.vcoul, a.C, n1, g) : make_double2(0.0, 0.0);
+        if (n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + m] = c;
+        int8_t sr[kSlices], si[kSlices], sn[kSlices];
+        split4(c.x, scale, sr);
+        split4(c.y, scale, si);
+        split4(-c.y, scale, sn);
+#pragma unroll
+        for (int s = 0; s < kSlices; ++s) {
+            p[2 * s][2 * b] = sr[s];
+            p[2 * s][2 * b + 1] = sn[s];
+            p[2 * s + 1][2 * b] = si[s];
+            p[2 * s + 1][2 * b + 1] = sr[s];
+        }
+    }
+    const int lane = (m & 31) + 32 * (q & 1), kh = 2 * ks + (q >> 1);
+    const size_t rb = m >> 5, nrb = (size_t)a.ldm / 32;
+#pragma unroll
+    for (int pl = 0; pl < kPlanesA; ++pl) {
+        int8_t *dst = a.as + ((pl * nrb + rb) * a.nkh + kh) * kFrag + 16 * lane;
+        *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
+    }
+}
+
+// aqsn slices: the same scheme over columns; slice s holds (ar, ai).
+struct SplitBArgs {
+    const float2 *aqsn;
+    const double *pmax_b;
+    int8_t *bs;              // [4][ldn/32][Kp/32][1024]
+    double *rb;              // [ldn] M / 127 of the column
+    int nb, C, ldn, nkh, chunks;
+};
+
+__global__ __launch_bounds__(256) void oz_split_b_kernel(SplitBArgs a)
+{
+    const int tid = threadIdx.x, q = tid >> 6;
+    const int col = blockIdx.x * 64 + (tid & 63);
+    const int ks = blockIdx.y;
+    double mx = 0.0;
+    for (int c = 0; c < a.chunks; ++c) mx = fmax(mx, a.pmax_b[(size_t)c * a.ldn + col]);
+    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
+    if (ks == 0 && q == 0) a.rb[col] = mx / 127.0;
+
+    int8_t p[kPlanesB][16];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int n1 = ks * kBandsPerStep + 8 * q + b;
+        const float2 v = (col < a.C && n1 < a.nb) ? a.aqsn[(size_t)n1 * a.C + col] : make_float2(0.f, 0.f);
+        int8_t sr[kSlices], si[kSlices];
+        split4((double)v.x, scale, sr);
+        split4((double)v.y, scale, si);
+#pragma unroll
+        for (int s = 0; s < kSlices; ++s) {
+            p[s][2 * b] = sr[s];
+            p[s][2 * b + 1] = si[s];
+        }
+    }
+    const int lane = (col & 31) + 32 * (q & 1), kh = 2 * ks + (q >> 1);
+    const size_t cb = col >> 5, ncb = (size_t)a.ldn / 32;
+#pragma unroll
+    for (int pl = 0; pl < kPlanesB; ++pl) {
+        int8_t *dst = a.bs + ((pl * ncb + cb) * a.nkh + kh) * kFrag + 16 * lane;
+        *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// int8 MFMA GEMM.  64x64 workgroup tile, 2x2 waves of one 32x32 block
+// each.  32x32x32 int8 MFMA: lane l supplies 16 bytes of row l%32 of A and
+// of column l%32 of B at the same K' positions; accumulator register j of
+// lane l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32.
+//
+// LDS ring of kStages k-steps; a stage holds the workgroup's 32 A fragments
+// ([plane][row block][k-half]) and 16 B fragments ([slice][column
+// block][k-half]), 48 KB.  Each wave moves 12 fragments per step with
+// direct global->LDS loads, two steps ahead of the step it computes.
+// Step s: wait for the wave's loads of step s (those of step s+1 may stay
+// in flight), barrier, issue the loads of step s+2 into the stage step s-1
+// used, issue the step's 24 fragment reads, then its 52 MFMAs in a fixed
+// order that keeps two MFMAs into one accumulator at least three apart
+// (scheduling barriers hold the order; each MFMA waits only for its own
+// operands).
+// ---------------------------------------------------------------------------
+constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 3, kAhead = kStages - 1;
+constexpr int kFragsA = kPlanesA * (kBM / 32) * 2, kFragsB = kPlanesB * (kBN / 32) * 2;
+constexpr int kFragsPerStage = kFragsA + kFragsB;          // 48
+constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 12
+
+// MFMA order of one step: (slice of A, slice of B, k-half); both variants
+// follow each entry.  Levels si + sj: 3 4 2 3 4 1 3 2 4 3 0 2 1, twice.
+struct PairStep { int si, sj, kh; };
+constexpr int kPairs = 13;
+constexpr PairStep kOrder[2 * kPairs] = {
+    {0, 3, 0}, {1, 3, 0}, {0, 2, 0}, {1, 2, 0}, {2, 2, 0}, {0, 1, 0}, {2, 1, 0},
+    {1, 1, 0}, {3, 1, 0}, {3, 0, 0}, {0, 0, 0}, {2, 0, 0}, {1, 0, 0},
+    {0, 3, 1}, {1, 3, 1}, {0, 2, 1}, {1, 2, 1}, {2, 2, 1}, {0, 1, 1}, {2, 1, 1},
+    {1, 1, 1}, {3, 1, 1}, {3, 0, 1}, {0, 0, 1}, {2, 0, 1}, {1, 0, 1}};
+
+struct GemmArgs {
+    const int8_t *as, *bs;
+    const double *ra, *rb;
+    double2 *bmat;           // [ldm][ldn] B in FP64
+    int ldm, ldn, nkh, tiles_m, tiles_n, group_m;
+};
+
+typedef int i16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ i16v mfma_i8(i4 a, i4 b, i16v c)
+{
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// Workgroup b -> tile.  Each XCD runs a contiguous range of the logical
+// order; the logical order walks bands of group_m row tiles, column by
+// column, so a band's coefficient slices stay in the XCD's L2 while the
+// aqsn slices stream past.
+__device__ __forceinline__ void oz_tile(int b, int tiles_m, int tiles_n, int group_m, int &tm, int &tn)
+{
+    const int total = tiles_m * tiles_n;
+    const int q = total / kXcds, r = total % kXcds, x = b % kXcds, i = b / kXcds;
+    const int t = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    const int band = t / (group_m * tiles_n);
+    const int first = band * group_m;
+    const int gm = min(tiles_m - first, group_m);
+    const int u = t - band * group_m * tiles_n;
+    tm = first + u % gm;
+    tn = u / gm;
+}
+
+__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
+{
+    __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    oz_tile(blockIdx.x, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
+    const int nks = a.nkh / 2;
+
+    // global source of this wave's fragment loads: stage fragment f is A
+    // (plane f/4, row block 2 tm + (f/2)%2, k-half f%2) for f < 32, else B
+    // (slice (f-32)/4, column block 2 tn + ((f-32)/2)%2, k-half f%2);
+    // consecutive k-steps are two fragments apart
+    const size_t nrb = (size_t)a.ldm / 32, ncb = (size_t)a.ldn / 32;
+    const int8_t *src[kLoadsPerWave];
+#pragma unroll
+    for (int k = 0; k < kLoadsPerWave; ++k) {
+        const int f = wave * kLoadsPerWave + k;
+        const int8_t *base;
+        if (f < kFragsA) {
+            const int pl = f >> 2, rb = 2 * tm + ((f >> 1) & 1), kh = f & 1;
+            base = a.as + (((size_t)pl * nrb + rb) * a.nkh + kh) * kFrag;
+        } else {
+            const int g = f - kFragsA, pl = g >> 2, cb = 2 * tn + ((g >> 1) & 1), kh = g & 1;
+            base = a.bs + (((size_t)pl * ncb + cb) * a.nkh + kh) * kFrag;
+        }
+        src[k] = base + 16 * lane;
+    }
+    auto issue = [&](int s) {
+        int8_t *dst = &lds[s % kStages][wave * kLoadsPerWave][0];
+#pragma unroll
+        for (int k = 0; k < kLoadsPerWave; ++k)
+            __builtin_amdgcn_global_load_lds(src[k] + (size_t)s * 2 * kFrag, dst + k * kFrag, 16, 0, 0);
+    };
+
+    i16v acc[kLevels][2];
+#pragma unroll
+    for (int L = 0; L < kLevels; ++L)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[L][v] = i16v{};
+
+    issue(0);
+    if (nks > 1) issue(1);
+    for (int s = 0; s < nks; ++s) {
+        // the wave's loads of step s are done once at most the 12 of step
+        // s+1 are outstanding; the barrier then covers every wave's loads.
+        // A plain s_barrier: __syncthreads would also drain step s+1.
+        if (s + 1 < nks) asm volatile("s_waitcnt vmcnt(12)\n\ts_barrier" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        if (s + kAhead < nks) issue(s + kAhead);
+        const int8_t *st = &lds[s % kStages][0][0];
+        i4 fa[kSlices][2][2], fb[kSlices][2];   // [slice][variant][k-half], [slice][k-half]
+#pragma unroll
+        for (int o = 0; o < 2 * kSlices; ++o) {   // in the order the MFMAs first use them
+            const int sl = o >> 1, kh = o & 1;
+            fb[3 - sl][kh] = *reinterpret_cast<const i4 *>(st + (kFragsA + (3 - sl) * 4 + wn * 2 + kh) * kFrag + 16 * lane);
+            fa[sl][0][kh] = *reinterpret_cast<const i4 *>(st + ((2 * sl) * 4 + wm * 2 + kh) * kFrag + 16 * lane);
+            fa[sl][1][kh] = *reinterpret_cast<const i4 *>(st + ((2 * sl + 1) * 4 + wm * 2 + kh) * kFrag + 16 * lane);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int o = 0; o < 2 * kPairs; ++o) {
+            const PairStep ps = kOrder[o];
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                acc[ps.si + ps.sj][v] = mfma_i8(fa[ps.si][v][ps.kh], fb[ps.sj][ps.kh], acc[ps.si + ps.sj][v]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+
+    // B = ra rb sum_L 254^-L acc_L in FP64 (int32 -> FP64 is exact)
+    const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0, 1.0 / 4162314256.0};
+    const int col = tn * kBN + wn * 32 + (lane & 31);
+    const double rbc = a.rb[col];
+    double rar[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) rar[j] = a.ra[tm * kBM + wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3)];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int row = tm * kBM + wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        double bre = 0.0, bim = 0.0;
+#pragma unroll
+        for (int L = kLevels - 1; L >= 0; --L) {
+            bre += W[L] * (double)acc[L][0][j];
+            bim += W[L] * (double)acc[L][1][j];
+        }
+        const double sc = rar[j] * rbc;
+        a.bmat[(size_t)row * a.ldn + col] = make_double2(bre * sc, bim * sc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pole/SX epilogue over every (my_igp, ig): B from bmat, Bo = sum over the
+// occupied bands in FP64, then the FP64 pole and SX factors; one partial
+// per workgroup and frequency.  Elements are dealt to threads in a fixed
+// grid-stride order, so the sums are reproducible.
+// ---------------------------------------------------------------------------
+struct PoleArgs {
+    const double2 *bmat;
+    const double2 *c_occ;    // [nv][ldm]
+    const float2 *aqsn, *I_eps, *wtilde;
+    const int *igp;
+    double2 *partials;       // [blocks][nw][2]
+    int nv, C, G, ldm, ldn;
+    PoleParams pp;
+};
+
+template <int NWC>
+__global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
+{
+    constexpr int NWAVES = 4;
+    __shared__ double red[NWAVES][4 * NWC];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool want_ssx = a.nv > 0;
+    const int64_t n = (int64_t)a.G * a.C;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int w0 = 0; w0 < a.pp.nw; w0 += NWC) {
+        double2 sach[NWC], sasx[NWC];
+#pragma unroll
+        for (int q = 0; q < NWC; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
+        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + tid; e < n; e += stride) {
+            const int m = (int)(e / a.C), ig = (int)(e - (int64_t)m * a.C);
+            const bool ok = a.igp[m] >= 0;
+            const double2 B = a.bmat[(size_t)m * a.ldn + ig];
+            double2 Bo = make_double2(0.0, 0.0);
+            for (int n1 = 0; n1 < a.nv; ++n1)
+                Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
+            pole_element<NWC>(a.pp, w0, ok, widen(a.wtilde[e]), widen(a.I_eps[e]), B, Bo, want_ssx, sach, sasx);
+        }
+#pragma unroll
+        for (int q = 0; q < NWC; ++q) {
+            double v0 = wave_sum(sach[q].x), v1 = wave_sum(sach[q].y);
+            double v2 = wave_sum(sasx[q].x), v3 = wave_sum(sasx[q].y);
+            if (lane == 0) {
+                red[wave][4 * q + 0] = v0;
+                red[wave][4 * q + 1] = v1;
+                red[wave][4 * q + 2] = v2;
+                red[wave][4 * q + 3] = v3;
+            }
+        }
+        __syncthreads();
+        if (tid < 4 * NWC) {
+            const int q = tid >> 2, comp = tid & 3;
+            const int iw = w0 + q;
+            if (iw < a.pp.nw) {
+                double sum = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
+                double *dst = reinterpret_cast<double *>(a.partials + ((size_t)blockIdx.x * a.pp.nw + iw) * 2);
+                dst[comp] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kPoleBlocks = 2048;
+
+}  // namespace
+
+void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
+{
+    const int nb = std::max(p->number_bands, 1);
+    L.ldm = div_up(p->ngpown, kBM) * kBM;
+    L.ldn = div_up(p->ncouls, kBN) * kBN;
+    L.Kp = div_up(2 * nb, kKStep) * kKStep;
+    L.tiles_m = L.ldm / kBM;
+    L.tiles_n = L.ldn / kBN;
+    L.chunks = div_up(nb, kMaxChunk);
+    L.off_as = ws.plan((size_t)kPlanesA * L.ldm * L.Kp);
+    L.off_bs = ws.plan((size_t)kPlanesB * L.ldn * L.Kp);
+    L.off_ea = ws.plan(sizeof(double) * (size_t)L.ldm);
+    L.off_eb = ws.plan(sizeof(double) * (size_t)L.ldn);
+    L.off_pmax = ws.plan(sizeof(double) * (size_t)L.chunks * (L.ldm + L.ldn));
+    L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
+    L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
+    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * kPoleBlocks);
+}
+
+int gpp_oz_check(const fk_gpp_problem *p)
+{
+    if (p->number_bands > kMaxBandsOz) {
+        set_error("gpp_kernel: the mixed-precision path takes at most %d bands (got %d)", kMaxBandsOz,
+                  p->number_bands);
+        return FK_ERR_UNSUPPORTED;
+    }
+    return FK_OK;
+}
+
+int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, Scratch &ws, const int *d_igp,
+                  const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
+                  double2 *d_ach, double2 *d_asx, hipStream_t s)
+{
+    const int nks = L.Kp / kKStep, nkh = L.Kp / kKHalf;
+    double *pmax = static_cast<double *>(ws.at(L.off_pmax));
+    MaxArgs ma;
+    ma.aqsm = aqsm; ma.aqsn = aqsn; ma.vcoul = vcoul; ma.igp = d_igp;
+    ma.pmax_a = pmax; ma.pmax_b = pmax + (size_t)L.chunks * L.ldm;
+    ma.nb = p->number_bands; ma.C = p->ncouls; ma.G = p->ngpown; ma.ldm = L.ldm; ma.ldn = L.ldn;
+    ma.chunks = L.chunks; ma.blocks_a = L.ldm / 64;
+    hipLaunchKernelGGL(oz_max_kernel, dim3(L.ldm / 64 + L.ldn / 64, L.chunks), dim3(256), 0, s, ma);
+    FK_HIP(hipGetLastError());
+
+    SplitAArgs sa;
+    sa.aqsm = aqsm; sa.vcoul = vcoul; sa.igp = d_igp; sa.pmax_a = ma.pmax_a;
+    sa.as = static_cast<int8_t *>(ws.at(L.off_as));
+    sa.ra = static_cast<double *>(ws.at(L.off_ea));
+    sa.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
+    sa.nb = p->number_bands; sa.nv = p->nvband; sa.C = p->ncouls; sa.G = p->ngpown; sa.ldm = L.ldm;
+    sa.nkh = nkh; sa.chunks = L.chunks;
+    hipLaunchKernelGGL(oz_split_a_kernel, dim3(L.ldm / 64, nks), dim3(256), 0, s, sa);
+    FK_HIP(hipGetLastError());
+
+    SplitBArgs sb;
+    sb.aqsn = aqsn; sb.pmax_b = ma.pmax_b;
+    sb.bs = static_cast<int8_t *>(ws.at(L.off_bs));
+    sb.rb = static_cast<double *>(ws.at(L.off_eb));
+    sb.nb = p->number_bands; sb.C = p->ncouls; sb.ldn = L.ldn; sb.nkh = nkh; sb.chunks = L.chunks;
+    hipLaunchKernelGGL(oz_split_b_kernel, dim3(L.ldn / 64, nks), dim3(256), 0, s, sb);
+    FK_HIP(hipGetLastError());
+
+    GemmArgs ga;
+    ga.as = sa.as; ga.bs = sb.bs; ga.ra = sa.ra; ga.rb = sb.rb;
+    ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
+    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = nkh;
+    ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
+    const int tiles = L.tiles_m * L.tiles_n;
+    KernelTimer timer(o->kernel_ms, s);
+    FK_TRY(timer.start());
+    hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(tiles), dim3(256), 0, s, ga);
+    FK_HIP(hipGetLastError());
+
+    PoleArgs pa;
+    pa.bmat = ga.bmat; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
+    pa.partials = static_cast<double2 *>(ws.at(L.off_part));
+    pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
+    pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
+    switch (std::min(p->nfreq, 4)) {
+    case 1: hipLaunchKernelGGL(gpp_oz_pole_kernel<1>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
+    case 2: hipLaunchKernelGGL(gpp_oz_pole_kernel<2>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
+    case 3: hipLaunchKernelGGL(gpp_oz_pole_kernel<3>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
+    default: hipLaunchKernelGGL(gpp_oz_pole_kernel<4>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
+    }
+    FK_HIP(hipGetLastError());
+    FK_TRY(timer.stop());
+    FK_TRY(gpp_reduce_launch(pa.partials, kPoleBlocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
+    return timer.finish();
+}
+
+}  // namespace fk

@@ -90,4 +90,21 @@ int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *a
                      const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
                      hipStream_t s);
 
+// Fixed-order sum of GPP per-tile partials [tiles][nw][2] into achtemp and
+// asxtemp (asxtemp scaled by asx_scale).
+int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
+                      double2 *asxtemp, hipStream_t s);
+
+// Mixed-precision GPP on int8 MFMA (complex64 inputs): workspace layout,
+// size limits, and the launch of its three kernels plus the reduction.
+struct OzLayout {
+    int ldm, ldn, Kp, tiles_m, tiles_n, chunks;
+    size_t off_as, off_bs, off_ea, off_eb, off_pmax, off_cocc, off_bmat, off_part;
+};
+void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L);
+int gpp_oz_check(const fk_gpp_problem *p);
+int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, Scratch &ws, const int *d_igp,
+                  const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
+                  double2 *d_ach, double2 *d_asx, hipStream_t s);
+
 }  // namespace fk
