@@ -24,7 +24,7 @@
 //   oz_split_a_kernel    coefficients -> int8 slices, FP64 occupied rows
 //   oz_split_b_kernel    aqsn -> int8 slices
 //   gpp_oz_gemm_kernel   int8 MFMA tile, operands staged through LDS by
-//                        direct global->LDS loads in a three-deep ring;
+//                        direct global->LDS loads in a five-deep ring;
 //                        B rebuilt in FP64 and stored
 //   gpp_oz_pole_kernel   per (my_igp, ig): occupied-band sum Bo in FP64, the
 //                        pole and SX factors in FP64, per-block partials
@@ -41,6 +41,7 @@
 #include "fk_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fk {
 namespace {
@@ -54,7 +55,7 @@ constexpr int kKStep = 64;      // K' per pipeline step = 32 bands = two MFMA k-
 constexpr int kKHalf = 32;      // K' per 32x32x32 int8 MFMA
 constexpr int kBandsPerStep = kKStep / 2;
 constexpr int kMaxChunk = 64;   // bands per partial maximum
-constexpr int kPlanesA = 2 * kSlices, kPlanesB = kSlices;
+constexpr int kPlanesA = kSlices, kPlanesB = kSlices;
 constexpr int kFrag = 1024;     // bytes per lane-ordered fragment
 constexpr int kMaxBandsOz = 16000;   // keeps the level-3 int32 sums (4 pairs) below 2^31
 
@@ -130,15 +131,15 @@ __global__ __launch_bounds__(256) void oz_max_kernel(MaxArgs a)
 // Coefficient slices.  Workgroup (x, y) = 64 rows x one k-step (32 bands);
 // thread t: row m = 64 x + t%64, bands 32 y + 8 q .. +7 with q = t/64, i.e.
 // k-half 2 y + q/2 and lane L = m%32 + 32 (q%2) of the fragment
-// (plane, m/32, k-half) of every plane.  Plane 2 s + v holds slice s of
-// the pairs (cr, -ci) for v = 0 and (ci, cr) for v = 1.
+// (slice, m/32, k-half) of every slice.  The stored pairs are (cr, -ci);
+// the GEMM derives (ci, cr) from them.
 // ---------------------------------------------------------------------------
 struct SplitAArgs {
     const float2 *aqsm;
     const double *vcoul;
     const int *igp;
     const double *pmax_a;
-    int8_t *as;              // [8][ldm/32][Kp/32][1024]
+    int8_t *as;              // [4][ldm/32][Kp/32][1024]
     double *ra;              // [ldm] M / 127 of the row
     double2 *c_occ;          // [nv][ldm]
     int nb, nv, C, G, ldm, nkh, chunks;
@@ -161,16 +162,13 @@ __global__ __launch_bounds__(256) void oz_split_a_kernel(SplitAArgs a)
         const int n1 = ks * kBandsPerStep + 8 * q + b;
         const double2 c = (g >= 0 && n1 < a.nb) ? coef(a.aqsm, a.vcoul, a.C, n1, g) : make_double2(0.0, 0.0);
         if (n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + m] = c;
-        int8_t sr[kSlices], si[kSlices], sn[kSlices];
+        int8_t sr[kSlices], sn[kSlices];
         split4(c.x, scale, sr);
-        split4(c.y, scale, si);
         split4(-c.y, scale, sn);
 #pragma unroll
         for (int s = 0; s < kSlices; ++s) {
-            p[2 * s][2 * b] = sr[s];
-            p[2 * s][2 * b + 1] = sn[s];
-            p[2 * s + 1][2 * b] = si[s];
-            p[2 * s + 1][2 * b + 1] = sr[s];
+            p[s][2 * b] = sr[s];
+            p[s][2 * b + 1] = sn[s];
         }
     }
     const int lane = (m & 31) + 32 * (q & 1), kh = 2 * ks + (q >> 1);
@@ -230,31 +228,35 @@ __global__ __launch_bounds__(256) void oz_split_b_kernel(SplitBArgs a)
 // of column l%32 of B at the same K' positions; accumulator register j of
 // lane l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32.
 //
-// LDS ring of kStages k-steps; a stage holds the workgroup's 32 A fragments
-// ([plane][row block][k-half]) and 16 B fragments ([slice][column
-// block][k-half]), 48 KB.  Each wave moves 12 fragments per step with
-// direct global->LDS loads, two steps ahead of the step it computes.
-// Step s: wait for the wave's loads of step s (those of step s+1 may stay
-// in flight), barrier, issue the loads of step s+2 into the stage step s-1
-// used, issue the step's 24 fragment reads, then its 52 MFMAs in a fixed
-// order that keeps two MFMAs into one accumulator at least three apart
-// (scheduling barriers hold the order; each MFMA waits only for its own
-// operands).
+// LDS ring of kStages k-steps; a stage holds the workgroup's 16 A fragments
+// ([slice][row block][k-half]) and 16 B fragments ([slice][column
+// block][k-half]), 32 KB.  Each wave moves 8 fragments per step with direct
+// global->LDS loads, issued four steps before the step that uses them.
+// A step is two halves (32 K' each).  A half runs its 26 MFMAs from one
+// register set and, between them at fixed positions, reads the next
+// half's 8 fragments from LDS into the other set and derives their (ci, cr)
+// pairs from the stored (cr, -ci) ones, so LDS traffic and the byte
+// shuffles overlap the MFMAs.  Between the halves of step s the wave waits
+// for its loads of step s+1 (later steps may stay in flight) and passes a
+// barrier that covers every wave's loads; the second half then issues the
+// loads of step s+4, spread over its MFMAs, into the stage step s-1 used.  Scheduling barriers hold the order; two
+// MFMAs into one accumulator are at least five apart.
 // ---------------------------------------------------------------------------
-constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 3, kAhead = kStages - 1;
+constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 5, kAhead = kStages - 1;
 constexpr int kFragsA = kPlanesA * (kBM / 32) * 2, kFragsB = kPlanesB * (kBN / 32) * 2;
-constexpr int kFragsPerStage = kFragsA + kFragsB;          // 48
-constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 12
+constexpr int kFragsPerStage = kFragsA + kFragsB;          // 32
+constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 8
 
-// MFMA order of one step: (slice of A, slice of B, k-half); both variants
-// follow each entry.  Levels si + sj: 3 4 2 3 4 1 3 2 4 3 0 2 1, twice.
-struct PairStep { int si, sj, kh; };
+// MFMA order of one half: (slice of A, slice of B); both variants follow
+// each entry.  Levels si + sj: 3 4 2 3 4 1 3 2 4 3 0 2 1.
+struct PairStep { int si, sj; };
 constexpr int kPairs = 13;
-constexpr PairStep kOrder[2 * kPairs] = {
-    {0, 3, 0}, {1, 3, 0}, {0, 2, 0}, {1, 2, 0}, {2, 2, 0}, {0, 1, 0}, {2, 1, 0},
-    {1, 1, 0}, {3, 1, 0}, {3, 0, 0}, {0, 0, 0}, {2, 0, 0}, {1, 0, 0},
-    {0, 3, 1}, {1, 3, 1}, {0, 2, 1}, {1, 2, 1}, {2, 2, 1}, {0, 1, 1}, {2, 1, 1},
-    {1, 1, 1}, {3, 1, 1}, {3, 0, 1}, {0, 0, 1}, {2, 0, 1}, {1, 0, 1}};
+constexpr int kMfmasHalf = 2 * kPairs;                     // 26
+constexpr PairStep kOrder[kPairs] = {{0, 3}, {1, 3}, {0, 2}, {1, 2}, {2, 2}, {0, 1}, {2, 1},
+                                     {1, 1}, {3, 1}, {3, 0}, {0, 0}, {2, 0}, {1, 0}};
+// positions in a half's MFMA stream: the 8 LDS reads follow MFMAs 0..7, the
+// 16 dword shuffles follow MFMAs kRotFirst .. kRotFirst + 15
+constexpr int kRotFirst = 9;
 
 struct GemmArgs {
     const int8_t *as, *bs;
@@ -268,6 +270,18 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ i16v mfma_i8(i4 a, i4 b, i16v c)
 {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// (cr, -ci) byte pairs -> (ci, cr) in one dword: negate the high byte of
+// each 16-bit half (two's complement within the half; the carry out of bit
+// 15 is dropped by the 16-bit lane), then swap the two bytes of each half.
+__device__ __forceinline__ int rotate_pair_dword(int x)
+{
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    const unsigned d = (unsigned)x ^ 0xFF00FF00u;
+    const u16x2 h = __builtin_bit_cast(u16x2, d) + u16x2{0x0100, 0x0100};
+    const unsigned n = __builtin_bit_cast(unsigned, h);
+    return (int)__builtin_amdgcn_perm(n, n, 0x02030001u);
 }
 
 // Workgroup b -> tile.  Each XCD runs a contiguous range of the logical
@@ -287,6 +301,23 @@ __device__ __forceinline__ void oz_tile(int b, int tiles_m, int tiles_n, int gro
     tn = u / gm;
 }
 
+// wait until at most n of this wave's fragment loads are outstanding, then
+// pass the workgroup barrier (n = 8 x the steps issued after the awaited one)
+__device__ __forceinline__ void wait_loads_barrier(int n)
+{
+    switch (n) {
+    case 24: asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory"); break;
+    case 16: asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)\n\ts_barrier" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); break;
+    }
+}
+
+struct HalfFrags {
+    i4 a[kSlices][2];        // [slice][variant]
+    i4 b[kSlices];           // [slice]
+};
+
 __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
@@ -298,8 +329,8 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
     const int nks = a.nkh / 2;
 
     // global source of this wave's fragment loads: stage fragment f is A
-    // (plane f/4, row block 2 tm + (f/2)%2, k-half f%2) for f < 32, else B
-    // (slice (f-32)/4, column block 2 tn + ((f-32)/2)%2, k-half f%2);
+    // (slice f/4, row block 2 tm + (f/2)%2, k-half f%2) for f < 16, else B
+    // (slice (f-16)/4, column block 2 tn + ((f-16)/2)%2, k-half f%2);
     // consecutive k-steps are two fragments apart
     const size_t nrb = (size_t)a.ldm / 32, ncb = (size_t)a.ldn / 32;
     const int8_t *src[kLoadsPerWave];
@@ -316,11 +347,27 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
         }
         src[k] = base + 16 * lane;
     }
-    auto issue = [&](int s) {
+    auto issue_one = [&](int s, int k) {
         int8_t *dst = &lds[s % kStages][wave * kLoadsPerWave][0];
+        __builtin_amdgcn_global_load_lds(src[k] + (size_t)s * 2 * kFrag, dst + k * kFrag, 16, 0, 0);
+    };
+    auto issue = [&](int s) {
 #pragma unroll
-        for (int k = 0; k < kLoadsPerWave; ++k)
-            __builtin_amdgcn_global_load_lds(src[k] + (size_t)s * 2 * kFrag, dst + k * kFrag, 16, 0, 0);
+        for (int k = 0; k < kLoadsPerWave; ++k) issue_one(s, k);
+    };
+    // LDS read #k (0..7) of a half's fragments (stage st, k-half kh): B slice
+    // 3 - k/2 for even k, A slice k/2 for odd k (the order the MFMAs first
+    // need them)
+    auto read_frag = [&](const int8_t *st, int kh, HalfFrags &f, int k) {
+        const int sl = k >> 1;
+        if ((k & 1) == 0)
+            f.b[3 - sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + (3 - sl) * 4 + wn * 2 + kh) * kFrag + 16 * lane);
+        else
+            f.a[sl][0] = *reinterpret_cast<const i4 *>(st + (sl * 4 + wm * 2 + kh) * kFrag + 16 * lane);
+    };
+    // shuffle #k (0..15): dword k%4 of A slice k/4
+    auto rotate = [&](HalfFrags &f, int k) {
+        f.a[k >> 2][1][k & 3] = rotate_pair_dword(f.a[k >> 2][0][k & 3]);
     };
 
     i16v acc[kLevels][2];
@@ -329,35 +376,59 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
 #pragma unroll
         for (int v = 0; v < 2; ++v) acc[L][v] = i16v{};
 
-    issue(0);
-    if (nks > 1) issue(1);
-    for (int s = 0; s < nks; ++s) {
-        // the wave's loads of step s are done once at most the 12 of step
-        // s+1 are outstanding; the barrier then covers every wave's loads.
-        // A plain s_barrier: __syncthreads would also drain step s+1.
-        if (s + 1 < nks) asm volatile("s_waitcnt vmcnt(12)\n\ts_barrier" ::: "memory");
-        else asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-        if (s + kAhead < nks) issue(s + kAhead);
-        const int8_t *st = &lds[s % kStages][0][0];
-        i4 fa[kSlices][2][2], fb[kSlices][2];   // [slice][variant][k-half], [slice][k-half]
-#pragma unroll
-        for (int o = 0; o < 2 * kSlices; ++o) {   // in the order the MFMAs first use them
-            const int sl = o >> 1, kh = o & 1;
-            fb[3 - sl][kh] = *reinterpret_cast<const i4 *>(st + (kFragsA + (3 - sl) * 4 + wn * 2 + kh) * kFrag + 16 * lane);
-            fa[sl][0][kh] = *reinterpret_cast<const i4 *>(st + ((2 * sl) * 4 + wm * 2 + kh) * kFrag + 16 * lane);
-            fa[sl][1][kh] = *reinterpret_cast<const i4 *>(st + ((2 * sl + 1) * 4 + wm * 2 + kh) * kFrag + 16 * lane);
-        }
+    // one half: MFMAs from `cur`; with NEXT, the following half's fragments
+    // (stage st, k-half kh) into `nxt`; with LOADS, the global->LDS loads of
+    // step `ls`, one after every other MFMA
+    auto half = [&](auto next_t, auto loads_t, const HalfFrags &cur, HalfFrags &nxt, const int8_t *st, int kh,
+                    int ls) {
+        constexpr bool NEXT = decltype(next_t)::value;
+        constexpr bool LOADS = decltype(loads_t)::value;
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int o = 0; o < 2 * kPairs; ++o) {
-            const PairStep ps = kOrder[o];
-#pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                acc[ps.si + ps.sj][v] = mfma_i8(fa[ps.si][v][ps.kh], fb[ps.sj][ps.kh], acc[ps.si + ps.sj][v]);
-                __builtin_amdgcn_sched_barrier(0);
+        for (int o = 0; o < kMfmasHalf; ++o) {
+            const PairStep ps = kOrder[o >> 1];
+            const int v = o & 1;
+            acc[ps.si + ps.sj][v] = mfma_i8(cur.a[ps.si][v], cur.b[ps.sj], acc[ps.si + ps.sj][v]);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (NEXT) {
+                if (o < 8) read_frag(st, kh, nxt, o);
+                if (o >= kRotFirst && o < kRotFirst + 16) rotate(nxt, o - kRotFirst);
             }
+            if constexpr (LOADS) {
+                if ((o & 1) && o < 2 * kLoadsPerWave) issue_one(ls, o >> 1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    using yes = std::integral_constant<bool, true>;
+    using no = std::integral_constant<bool, false>;
+
+    HalfFrags h0, h1;
+#pragma unroll
+    for (int s = 0; s < kAhead; ++s)
+        if (s < nks) issue(s);
+    wait_loads_barrier(8 * min(kAhead - 1, nks - 1));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) read_frag(&lds[0][0][0], 0, h0, k);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) rotate(h0, k);
+    auto stage = [&](int s) { return &lds[s % kStages][0][0]; };
+    int s = 0;
+    // steady state: step s+1 is awaited with steps s+2, s+3 in flight, and
+    // step s+4 exists
+    for (; s + kAhead < nks; ++s) {
+        half(yes{}, no{}, h0, h1, stage(s), 1, 0);                    // (s, 0); reads (s, 1)
+        asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
+        half(yes{}, yes{}, h1, h0, stage(s + 1), 0, s + kAhead);      // (s, 1); reads (s+1, 0)
     }
+    // the last steps: fewer loads in flight, none issued
+    for (; s + 1 < nks; ++s) {
+        half(yes{}, no{}, h0, h1, stage(s), 1, 0);
+        wait_loads_barrier(8 * min(2, nks - 2 - s));
+        half(yes{}, no{}, h1, h0, stage(s + 1), 0, 0);
+    }
+    half(yes{}, no{}, h0, h1, stage(s), 1, 0);
+    half(no{}, no{}, h1, h0, stage(s), 0, 0);
 
     // B = ra rb sum_L 254^-L acc_L in FP64 (int32 -> FP64 is exact)
     const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0, 1.0 / 4162314256.0};
