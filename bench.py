@@ -13,10 +13,12 @@ cpu_baseline = the reference's own gppKer program (oracle/_ref, built from
 the reference sources) on a bounded band sample of the same shape, timed on
 this host's cores; the CPU oracle restatement is used if it is absent.
 
-Two more objects ride on the same line: "mixed" (the complex64-storage GPP
-form on the same shape) and "ff" (the FF frequency contraction at
+More objects ride on the same line: "mixed" (the complex64-storage GPP
+form on the same shape), "ff" (the FF frequency contraction at
 number_bands 256, ncouls = ngpown = 4096, nFreq 32, FP64 MFMA, with its own
-roofline against the FP64 MFMA peak).
+roofline against the FP64 MFMA peak) and "batched" (64 uneven GPP problems
+in one persistent graph launch, FP64 and mixed, tile configuration chosen
+by the library's autotuner).
 """
 import argparse
 import json
@@ -184,6 +186,69 @@ def run_ff(fk, torch, steps, warmup, seed=4):
                          "alg_flops_per_launch": flops, "kernel_ms": kernel_ms}}
 
 
+def batched_shapes(count=64, seed=3):
+    """number_bands ~U{128..1024}, ncouls ~U{4096..32768} rounded to 64,
+    ngpown = ncouls/20, nvband ~U{2..16}, frequencies ~U{3..11}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nb = int(rng.integers(128, 1025))
+        C = int(rng.integers(4096, 32769)) // 64 * 64
+        out.append((nb, int(rng.integers(2, 17)), C, C // 20, int(rng.integers(3, 12))))
+    return out
+
+
+def run_batched(fk, torch, steps):
+    """64 problems as one batch, FP64 then mixed (complex64 storage, FP32
+    MFMA contraction); inputs resident in HBM."""
+    shapes = batched_shapes()
+    updates = float(sum(nb * G * C * nw for nb, nv, C, G, nw in shapes))
+    dev = torch.device("cuda:0")
+    result = {"workload": "64 GPP problems: number_bands U{128..1024}, ncouls U{4096..32768}/64*64, "
+                          "ngpown ncouls/20, nvband U{2..16}, frequencies U{3..11}, one graph launch",
+              "updates": updates}
+    keep = {}
+    for prec, cdt in (("fp64", torch.complex128), ("mixed", torch.complex64)):
+        log(f"batched {prec}: generating 64 problems ({updates:.3e} updates)")
+        probs = []
+        for k, (nb, nv, C, G, nw) in enumerate(shapes):
+            seed = 100 + k
+            t = {}
+            for name, rows, re in (("aqsm", nb, (-0.5, 0.5)), ("aqsn", nb, (-0.5, 0.5)), ("I_eps", G, (-0.5, 0.5)),
+                                   ("wtilde", G, (0.5, 2.0))):
+                t[name] = torch.empty((rows, C), dtype=cdt, device=dev)
+                fk.fill_complex(t[name], seed, STREAMS[name], re, (-0.01, 0.01) if name == "wtilde" else (-0.5, 0.5))
+            t["vcoul"] = torch.empty(C, dtype=torch.float64, device=dev)
+            fk.fill_real(t["vcoul"], seed, STREAMS["vcoul"], 0.1, 10.0)
+            inv, indinv = index_maps(C, G)
+            probs.append(dict(nb=nb, nv=nv, ncouls=C, ngpown=G, wx=wx_array(nw), inv_igp_index=inv, indinv=indinv, **t))
+        torch.cuda.synchronize()
+        log(f"batched {prec}: create (autotuning)")
+        t0 = time.perf_counter()
+        b = fk.GppBatch(probs, precision=prec, config=0)
+        create_s = time.perf_counter() - t0
+        b.run()
+        torch.cuda.synchronize()
+        log(f"batched {prec}: config {b.config}, timing {steps} runs")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ach, asx, _ = b.run()
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        keep[prec] = [a.cpu().numpy() for a in ach]
+        result[prec] = {"value": updates / (ms * 1e-3), "unit": "updates/s", "ms_per_step": ms, "config": b.config,
+                        "create_s": create_s, "workspace_bytes": b.workspace_bytes}
+        log(f"batched {prec}: {ms:.2f} ms per batch, {updates / (ms * 1e-3):.3e} updates/s")
+        b.close()
+        del probs
+        torch.cuda.empty_cache()
+    err = max(float(np.max(np.abs(m - f)) / np.max(np.abs(f))) for m, f in zip(keep["mixed"], keep["fp64"]))
+    result["mixed"]["speedup_vs_fp64"] = result["fp64"]["ms_per_step"] / result["mixed"]["ms_per_step"]
+    result["mixed"]["max_rel_err_vs_fp64_achtemp"] = err
+    result["mixed"]["error_note"] = "includes the rounding of the inputs to complex64"
+    return result
+
+
 def cpu_baseline(bands, threads):
     """The reference's own gppKer (oracle/_ref) on `bands` bands of the
     512/2/32768/20 shape, OMP threads = `threads`."""
@@ -228,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ff", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -281,6 +347,8 @@ def main():
         log(f"mixed: {m['ms_per_step']:.3f} ms/step ({line['mixed']['speedup_vs_fp64']:.2f}x vs fp64)")
     if not args.no_ff:
         line["ff"] = run_ff(fk, torch, max(3, args.steps // 4), 1)
+    if not args.no_batched:
+        line["batched"] = run_batched(fk, torch, 3)
     if args.calibrate_copy:
         n = args.calibrate_copy << 20
         src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")

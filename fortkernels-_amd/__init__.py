@@ -31,7 +31,9 @@ COP = {"mul": 0, "conj": 1, "abs2": 2, "abs": 3, "scale": 4, "fma": 5, "fms": 6,
 # every extern "C" symbol of include/fortkernels.h
 EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_kernel",
            "ff_kernel_workspace_size", "fk_last_error", "fk_abi_version", "fk_build_target",
-           "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe"]
+           "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe",
+           "gpp_batch_create", "gpp_batch_run", "gpp_batch_config", "gpp_batch_workspace_bytes",
+           "gpp_batch_destroy", "gpp_kernel_batched"]
 
 
 class FkError(RuntimeError):
@@ -102,6 +104,19 @@ def _load():
     L.fk_complex_apply.argtypes = [I32, I64, P, P, P, P, I32]
     L.fk_mfma_probe.restype = ctypes.c_int
     L.fk_mfma_probe.argtypes = [I32, P]
+    PP = ctypes.POINTER(ctypes.POINTER(GppProblem))
+    L.gpp_batch_create.restype = ctypes.c_int
+    L.gpp_batch_create.argtypes = [I32, PP, ctypes.POINTER(Options), ctypes.POINTER(P)]
+    L.gpp_batch_run.restype = ctypes.c_int
+    L.gpp_batch_run.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), P, ctypes.POINTER(ctypes.c_float)]
+    L.gpp_batch_config.restype = ctypes.c_int
+    L.gpp_batch_config.argtypes = [P]
+    L.gpp_batch_workspace_bytes.restype = SZ
+    L.gpp_batch_workspace_bytes.argtypes = [P]
+    L.gpp_batch_destroy.restype = None
+    L.gpp_batch_destroy.argtypes = [P]
+    L.gpp_kernel_batched.restype = ctypes.c_int
+    L.gpp_kernel_batched.argtypes = [I32, PP, ctypes.POINTER(Options), ctypes.POINTER(P), ctypes.POINTER(P)]
     return L
 
 
@@ -210,6 +225,73 @@ def gpp_problem(p, **kw):
     return gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, p.aqsm, p.aqsn, p.I_eps, p.wtilde, p.vcoul,
                       p.inv_igp_index, p.indinv, to1=p.to1, limittwo=p.limittwo, limitone=p.limitone,
                       occ=p.occ, **kw)
+
+
+# --------------------------------------------------------------------------
+# batched GPP
+# --------------------------------------------------------------------------
+class GppBatch:
+    """A batch of GPP problems on the device (gpp_batch_create).
+
+    problems: sequence of dicts with the gpp_kernel arguments: nb, nv, ncouls,
+    ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul (torch device tensors),
+    inv_igp_index, indinv and optionally to1, limittwo, limitone, occ.
+    config 0 lets the library time every tile configuration on the batch."""
+
+    def __init__(self, problems, precision="fp64", config=0, mask=True):
+        self._keep = []
+        structs = []
+        for p in problems:
+            wx = np.ascontiguousarray(p["wx"], np.float64)
+            inv = np.ascontiguousarray(p["inv_igp_index"], np.int32)
+            ind = np.ascontiguousarray(p["indinv"], np.int32)
+            self._keep += [wx, inv, ind, p]
+            structs.append(GppProblem(int(p["nb"]), int(p["nv"]), int(p["ncouls"]), int(p["ngpown"]), int(wx.shape[0]),
+                                      wx.ctypes.data, _ptr(p["aqsm"]), _ptr(p["aqsn"]), _ptr(p["I_eps"]),
+                                      _ptr(p["wtilde"]), _ptr(p["vcoul"]), inv.ctypes.data, ind.ctypes.data,
+                                      float(p.get("to1", 1e-6)), float(p.get("limittwo", 0.25)),
+                                      float(p.get("limitone", 2.5e5)), float(p.get("occ", 1.0))))
+        self.n = len(structs)
+        self.nw = [s.nfreq for s in structs]
+        self._structs = structs
+        arr = (ctypes.POINTER(GppProblem) * self.n)(*[ctypes.pointer(s) for s in structs])
+        opt = default_options()
+        opt.memory = FK_MEM_DEVICE
+        opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+        opt.mask = 1 if mask else 0
+        opt.config = int(config)
+        h = ctypes.c_void_p()
+        _check(lib.gpp_batch_create(self.n, arr, ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def config(self):
+        return lib.gpp_batch_config(self._h)
+
+    @property
+    def workspace_bytes(self):
+        return int(lib.gpp_batch_workspace_bytes(self._h))
+
+    def run(self, stream=None, time_it=False, out=None):
+        """Returns ([achtemp_i], [asxtemp_i], ms|None) as torch device tensors."""
+        import torch
+        if out is None:
+            out = ([torch.empty(w, dtype=torch.complex128, device="cuda") for w in self.nw],
+                   [torch.empty(w, dtype=torch.complex128, device="cuda") for w in self.nw])
+        ach, asx = out
+        pa = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in ach])
+        px = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in asx])
+        ms = ctypes.c_float(-1.0)
+        _check(lib.gpp_batch_run(self._h, pa, px, _stream_handle(stream), ctypes.byref(ms) if time_it else None))
+        return ach, asx, (float(ms.value) if time_it else None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.gpp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 # --------------------------------------------------------------------------

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
 
 namespace fk {
 namespace {
@@ -171,30 +172,27 @@ struct Frags {
     E a[MB], b[NB];
 };
 
-template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
-__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs<E> a)
+// LDS of one workgroup: per-wave reduction slots and the occupied-band
+// partial sums, one slot per (block, register, lane)
+template <class E, int MB, int NB, int NWAVES, int NWC>
+struct TileLds {
+    double red[NWAVES][4 * NWC];
+    E occ[NWAVES][MB * NB * 4 * 64];
+};
+
+// One (tm, tn) tile of one problem: GEMM, epilogue, partials.  Ends with
+// a workgroup barrier, so the LDS can be reused for the next tile.
+template <class E, int MB, int NB, int P, int WGM, int WGN, int NWC>
+__device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, TileLds<E, MB, NB, WGM * WGN, NWC> &sh)
 {
     using A4 = typename Acc<E>::type;
     constexpr int NWAVES = WGM * WGN;
     constexpr int WM = MB * 16, WN = NB * 16, BM = WM * WGM, BN = WN * WGN;
-
-    __shared__ double red[NWAVES][4 * NWC];
-    // occupied-band partial sums, one slot per (block, register, lane)
-    __shared__ E occ_lds[NWAVES][MB * NB * 4 * 64];
+    auto &red = sh.red;
+    auto &occ_lds = sh.occ;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WGM, wn = wave / WGM;
-
-    // The second workgroup slot of each CU starts about half a tile late, so
-    // that one slot's epilogue (VALU, memory) overlaps the other's MFMA loop;
-    // later workgroups inherit the offset.  The first slot runs alone at the
-    // full MFMA rate meanwhile.  The wait is bounded by the tick count.
-    if ((int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
-        const uint64_t t0 = wall_clock64();
-        while (wall_clock64() - t0 < (uint64_t)a.stagger_ticks) __builtin_amdgcn_s_sleep(32);
-    }
-    int tm, tn;
-    tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int rowbase = m0 + wm * WM, colbase = n0 + wn * WN;
     const int nmax = a.C - 1;
@@ -345,6 +343,47 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
             }
         }
         __syncthreads();
+    }
+}
+
+// One problem, one tile per workgroup.
+template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
+__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs<E> a)
+{
+    __shared__ TileLds<E, MB, NB, WGM * WGN, NWC> sh;
+    // The second workgroup slot of each CU starts about half a tile late, so
+    // that one slot's epilogue (VALU, memory) overlaps the other's MFMA loop;
+    // later workgroups inherit the offset.  The first slot runs alone at the
+    // full MFMA rate meanwhile.  The wait is bounded by the tick count.
+    if ((int)blockIdx.x >= a.stagger_lo && (int)blockIdx.x < a.stagger_hi) {
+        const uint64_t t0 = wall_clock64();
+        while (wall_clock64() - t0 < (uint64_t)a.stagger_ticks) __builtin_amdgcn_s_sleep(32);
+    }
+    int tm, tn;
+    tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
+    gpp_tile<E, MB, NB, P, WGM, WGN, NWC>(a, tm, tn, sh);
+}
+
+// Many problems in one launch: a fixed grid of workgroups takes tiles from a
+// shared counter, in the order of the tile table (largest K first), until
+// the table is exhausted.  tiles[t] = (problem, tm, tn).
+struct BatchTile { int prob, tm, tn; };
+
+template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
+__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_batch_kernel(const GppArgs<E> *probs,
+                                                                         const BatchTile *tiles, int ntiles,
+                                                                         int *counter)
+{
+    __shared__ TileLds<E, MB, NB, WGM * WGN, NWC> sh;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = next;
+        __syncthreads();
+        if (t >= ntiles) break;
+        const BatchTile bt = tiles[t];
+        gpp_tile<E, MB, NB, P, WGM, WGN, NWC>(probs[bt.prob], bt.tm, bt.tn, sh);
     }
 }
 
@@ -651,6 +690,352 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Batched problems: one coefficient launch, one persistent tile launch and
+// one reduction launch for the whole batch (captured once into a graph).
+// ---------------------------------------------------------------------------
+template <class E>
+struct CoefBatch {
+    const E *aqsm;
+    const double *vcoul;
+    const int *igp;          // resolved column map in device memory
+    E *c;
+    int nb, nv, nvpad, C, G, Kpad, ldc;
+    int64_t begin;           // first element of this problem in the flat index
+};
+
+template <class E>
+__global__ void gpp_coef_batch_kernel(const CoefBatch<E> *cb, int nprob, int64_t total)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        int lo = 0, hi = nprob - 1;            // last problem with begin <= idx
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cb[mid].begin <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const CoefBatch<E> &a = cb[lo];
+        const int64_t local = idx - a.begin;
+        const int kr = (int)(local / a.ldc), m = (int)(local % a.ldc);
+        const int g = m < a.G ? a.igp[m] : -1;
+        double2 v = make_double2(0.0, 0.0);
+        const int band = band_of(kr, a.nb, a.nv, a.nvpad);
+        if (band >= 0 && g >= 0) {
+            const double2 am = widen(a.aqsm[(size_t)band * a.C + g]);
+            const double sc = 0.5 * a.vcoul[g];
+            v = make_double2(am.x * sc, -am.y * sc);
+        }
+        E *dst = a.c + (size_t)kr * a.ldc + m;
+        if constexpr (sizeof(E) == sizeof(double2)) *dst = v;
+        else *dst = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+struct ReduceBatch {
+    const double2 *partials; // [tiles][nw][2]
+    int tiles, nw;
+    double asx_scale;
+};
+
+// blockIdx.y = problem, blockIdx.x = 2 iw + which; the same fixed-order sum
+// as gpp_reduce_kernel; out[p][0][iw] = achtemp, out[p][1][iw] = asxtemp
+__global__ __launch_bounds__(256) void gpp_reduce_batch_kernel(const ReduceBatch *rb, double2 *out)
+{
+    __shared__ double s_re[256], s_im[256];
+    const ReduceBatch &a = rb[blockIdx.y];
+    const int iw = blockIdx.x >> 1, which = blockIdx.x & 1;
+    if (iw >= a.nw) return;
+    double re = 0.0, im = 0.0;
+    for (int t = threadIdx.x; t < a.tiles; t += 256) {
+        const double2 v = a.partials[((size_t)t * a.nw + iw) * 2 + which];
+        re += v.x;
+        im += v.y;
+    }
+    s_re[threadIdx.x] = re;
+    s_im[threadIdx.x] = im;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_re[threadIdx.x] += s_re[threadIdx.x + w];
+            s_im[threadIdx.x] += s_im[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double sc = which ? a.asx_scale : 1.0;
+        out[((size_t)blockIdx.y * 2 + which) * kMaxFreq + iw] = make_double2(s_re[0] * sc, s_im[0] * sc);
+    }
+}
+
+__global__ void set_int_kernel(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+// ach[p][iw] = out[p][0][iw], asx[p][iw] = out[p][1][iw] for iw < nw[p]
+__global__ void batch_scatter_kernel(const double2 *out, const int *nw, double2 *const *ach, double2 *const *asx)
+{
+    const int p = blockIdx.x, iw = threadIdx.x;
+    if (iw < nw[p]) {
+        ach[p][iw] = out[((size_t)p * 2) * kMaxFreq + iw];
+        asx[p][iw] = out[((size_t)p * 2 + 1) * kMaxFreq + iw];
+    }
+}
+
+constexpr int kBatchNwc = 3;   // frequencies per epilogue pass in batches
+
+template <class E>
+void launch_batch(int cfg, int grid, hipStream_t s, const GppArgs<E> *probs, const BatchTile *tiles, int ntiles,
+                  int *counter)
+{
+    switch (cfg) {
+    case 0:
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 4, 1, 4, 1, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
+                           tiles, ntiles, counter);
+        break;
+    case 1:
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 2, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
+                           tiles, ntiles, counter);
+        break;
+    case 2:
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 4, 2, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
+                           tiles, ntiles, counter);
+        break;
+    default:
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
+                           tiles, ntiles, counter);
+        break;
+    }
+}
+
+// Host side of a batch: the layout of every problem, the device workspace,
+// the tile table for the chosen configuration and the captured graph.
+struct BatchProb {
+    int nb, nv, C, G, nw, Kpad, nvpad, ldc, max_tiles;
+    size_t off_c, off_igp, off_part;
+};
+
+struct GppBatch {
+    int n = 0, mixed = 0, cfg = -1, device = 0, ncu = 0, ntiles = 0, max_tiles_total = 0;
+    std::vector<fk_gpp_problem> src;
+    std::vector<BatchProb> prob;
+    void *mem = nullptr;
+    size_t bytes = 0, off_args = 0, off_tiles = 0, off_counter = 0, off_out = 0, off_coef = 0, off_red = 0,
+           off_nw = 0, off_ptrs = 0;
+    int64_t coef_total = 0;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    fk_options opt;
+
+    char *at(size_t off) const { return static_cast<char *>(mem) + off; }
+    size_t elem() const { return mixed ? sizeof(float2) : sizeof(double2); }
+
+    ~GppBatch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cs) {
+            (void)hipStreamSynchronize(cs);
+            (void)hipStreamDestroy(cs);
+        }
+        if (mem) (void)hipFree(mem);
+    }
+
+    // GppArgs and tile table of configuration `c`, uploaded on the batch's stream
+    template <class E>
+    int prepare(int c)
+    {
+        const TileCfg &t = kCfgs[c];
+        std::vector<GppArgs<E>> args(n);
+        std::vector<BatchTile> tiles;
+        std::vector<std::pair<std::pair<int, int>, int>> order;   // ((-nks, -nw), problem)
+        for (int i = 0; i < n; ++i) {
+            const BatchProb &b = prob[i];
+            const fk_gpp_problem &p = src[i];
+            GppArgs<E> &a = args[i];
+            a.c = reinterpret_cast<const E *>(at(b.off_c));
+            a.aqsn = static_cast<const E *>(p.aqsntemp);
+            a.I_eps = static_cast<const E *>(p.I_eps_array);
+            a.wtilde = static_cast<const E *>(p.wtilde_array);
+            a.igp = reinterpret_cast<const int *>(at(b.off_igp));
+            a.partials = reinterpret_cast<double2 *>(at(b.off_part));
+            a.nb = b.nb; a.nv = b.nv; a.C = b.C; a.G = b.G; a.nw = b.nw;
+            a.ldc = b.ldc; a.Kpad = b.Kpad; a.nvpad = b.nvpad; a.nks = b.Kpad / 4;
+            a.tiles_m = b.ldc / t.bm;
+            a.tiles_n = div_up(b.C, t.bn);
+            a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
+            a.pp = make_pole_params(p.wx, p.nfreq, p.limittwo, p.limitone, p.to1, opt.mask != 0);
+            if (b.nb > 0) order.push_back({{-a.nks, -b.nw}, i});
+        }
+        std::stable_sort(order.begin(), order.end());
+        for (const auto &o : order) {
+            const GppArgs<E> &a = args[o.second];
+            for (int tn = 0; tn < a.tiles_n; ++tn)
+                for (int tm = 0; tm < a.tiles_m; ++tm) tiles.push_back({o.second, tm, tn});
+        }
+        ntiles = (int)tiles.size();
+        std::vector<ReduceBatch> red(n);
+        for (int i = 0; i < n; ++i) {
+            red[i].partials = args[i].partials;
+            red[i].tiles = prob[i].nb > 0 ? args[i].tiles_m * args[i].tiles_n : 0;
+            red[i].nw = prob[i].nw;
+            red[i].asx_scale = 2.0 * src[i].occ;
+        }
+        FK_HIP(hipMemcpyAsync(at(off_args), args.data(), sizeof(GppArgs<E>) * n, hipMemcpyHostToDevice, cs));
+        if (ntiles)
+            FK_HIP(hipMemcpyAsync(at(off_tiles), tiles.data(), sizeof(BatchTile) * ntiles, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipMemcpyAsync(at(off_red), red.data(), sizeof(ReduceBatch) * n, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipStreamSynchronize(cs));
+        cfg = c;
+        return FK_OK;
+    }
+
+    // the launches of one run
+    template <class E>
+    int enqueue(hipStream_t s)
+    {
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<int *>(at(off_counter)), 0);
+        const int cblocks = (int)std::min<int64_t>((coef_total + 255) / 256, 1 << 16);
+        hipLaunchKernelGGL(gpp_coef_batch_kernel<E>, dim3(std::max(cblocks, 1)), dim3(256), 0, s,
+                           reinterpret_cast<const CoefBatch<E> *>(at(off_coef)), n, coef_total);
+        launch_batch<E>(cfg, ncu * kCfgs[cfg].minw, s, reinterpret_cast<const GppArgs<E> *>(at(off_args)),
+                        reinterpret_cast<const BatchTile *>(at(off_tiles)), ntiles,
+                        reinterpret_cast<int *>(at(off_counter)));
+        hipLaunchKernelGGL(gpp_reduce_batch_kernel, dim3(2 * kMaxFreq, n), dim3(256), 0, s,
+                           reinterpret_cast<const ReduceBatch *>(at(off_red)), reinterpret_cast<double2 *>(at(off_out)));
+        FK_HIP(hipGetLastError());
+        return FK_OK;
+    }
+
+    template <class E>
+    int build(int config)
+    {
+        // layout: Kpad a multiple of 48 and ldc of 128 serve every tile configuration
+        Scratch plan;
+        std::vector<CoefBatch<E>> coef(n);
+        off_args = plan.plan(sizeof(GppArgs<E>) * n);
+        off_red = plan.plan(sizeof(ReduceBatch) * n);
+        off_coef = plan.plan(sizeof(CoefBatch<E>) * n);
+        off_nw = plan.plan(sizeof(int) * n);
+        off_ptrs = plan.plan(sizeof(void *) * 2 * n);
+        off_out = plan.plan(sizeof(double2) * 2 * kMaxFreq * n);
+        off_counter = plan.plan(sizeof(int));
+        max_tiles_total = 0;
+        coef_total = 0;
+        for (int i = 0; i < n; ++i) {
+            BatchProb &b = prob[i];
+            const fk_gpp_problem &p = src[i];
+            b.nb = p.number_bands; b.nv = p.nvband; b.C = p.ncouls; b.G = p.ngpown; b.nw = p.nfreq;
+            b.nvpad = div_up(b.nv, 4) * 4;
+            b.Kpad = div_up(std::max(b.nvpad + b.nb - b.nv, 1), 48) * 48;
+            b.ldc = div_up(b.G, 128) * 128;
+            b.max_tiles = 0;
+            for (int c = 0; c < kNumCfgs; ++c)
+                b.max_tiles = std::max(b.max_tiles, (b.ldc / kCfgs[c].bm) * div_up(b.C, kCfgs[c].bn));
+            max_tiles_total += b.nb > 0 ? b.max_tiles : 0;
+            b.off_c = plan.plan(elem() * (size_t)b.Kpad * b.ldc);
+            b.off_igp = plan.plan(sizeof(int) * (size_t)b.G);
+            b.off_part = plan.plan(sizeof(double2) * 2 * (size_t)b.nw * std::max(b.max_tiles, 1));
+            coef[i].begin = coef_total;
+            coef_total += (int64_t)b.Kpad * b.ldc;
+        }
+        off_tiles = plan.plan(sizeof(BatchTile) * (size_t)std::max(max_tiles_total, 1));
+        bytes = plan.total();
+        if (hipMalloc(&mem, bytes) != hipSuccess) {
+            mem = nullptr;
+            set_error("gpp_batch_create: hipMalloc(%zu) failed", bytes);
+            return FK_ERR_NOMEM;
+        }
+        std::vector<int> nw(n);
+        for (int i = 0; i < n; ++i) {
+            const BatchProb &b = prob[i];
+            const fk_gpp_problem &p = src[i];
+            std::vector<int32_t> cols;
+            FK_TRY(resolve_columns(p.ncouls, p.ngpown, p.inv_igp_index, p.indinv, cols));
+            FK_HIP(hipMemcpyAsync(at(b.off_igp), cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, cs));
+            CoefBatch<E> &c = coef[i];
+            c.aqsm = static_cast<const E *>(p.aqsmtemp);
+            c.vcoul = p.vcoul;
+            c.igp = reinterpret_cast<const int *>(at(b.off_igp));
+            c.c = reinterpret_cast<E *>(at(b.off_c));
+            c.nb = b.nb; c.nv = b.nv; c.nvpad = b.nvpad; c.C = b.C; c.G = b.G; c.Kpad = b.Kpad; c.ldc = b.ldc;
+            nw[i] = b.nw;
+            FK_HIP(hipStreamSynchronize(cs));   // cols leaves scope
+        }
+        FK_HIP(hipMemcpyAsync(at(off_coef), coef.data(), sizeof(CoefBatch<E>) * n, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipMemcpyAsync(at(off_nw), nw.data(), sizeof(int) * n, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipStreamSynchronize(cs));
+
+        if (config > 0) {
+            FK_TRY(prepare<E>(config - 1));
+        } else {
+            // autotuner: time one run of each configuration (after a warm-up
+            // run) on this batch and keep the fastest
+            hipEvent_t e0, e1;
+            FK_HIP(hipEventCreate(&e0));
+            FK_HIP(hipEventCreate(&e1));
+            int best = 1;
+            float best_ms = 1e30f;
+            int rc = FK_OK;
+            for (int c = 0; c < kNumCfgs && rc == FK_OK; ++c) {
+                rc = prepare<E>(c);
+                if (rc == FK_OK) rc = enqueue<E>(cs);
+                if (rc == FK_OK && hipEventRecord(e0, cs) != hipSuccess) rc = FK_ERR_HIP;
+                if (rc == FK_OK) rc = enqueue<E>(cs);
+                if (rc == FK_OK && hipEventRecord(e1, cs) != hipSuccess) rc = FK_ERR_HIP;
+                float ms = 0.f;
+                if (rc == FK_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = FK_ERR_HIP;
+                if (rc == FK_OK && ms < best_ms) { best_ms = ms; best = c; }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc != FK_OK) { set_error("gpp_batch_create: autotuning failed"); return rc; }
+            FK_TRY(prepare<E>(best));
+        }
+        FK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue<E>(cs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        graph = g;
+        if (rc != FK_OK) return rc;
+        if (e != hipSuccess) { set_error("gpp_batch_create: graph capture: %s", hipGetErrorString(e)); return FK_ERR_HIP; }
+        FK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return FK_OK;
+    }
+
+    int run(fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, hipStream_t s, float *ms)
+    {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ms) {
+            FK_HIP(hipEventCreate(&e0));
+            FK_HIP(hipEventCreate(&e1));
+            FK_HIP(hipEventRecord(e0, s));
+        }
+        FK_HIP(hipGraphLaunch(exec, s));
+        if (ms) FK_HIP(hipEventRecord(e1, s));
+        std::vector<void *> ptrs(2 * n);
+        for (int i = 0; i < n; ++i) {
+            ptrs[i] = achtemp[i];
+            ptrs[n + i] = asxtemp[i];
+        }
+        FK_HIP(hipMemcpyAsync(at(off_ptrs), ptrs.data(), sizeof(void *) * 2 * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(batch_scatter_kernel, dim3(n), dim3(kMaxFreq), 0, s, reinterpret_cast<const double2 *>(at(off_out)),
+                           reinterpret_cast<const int *>(at(off_nw)), reinterpret_cast<double2 *const *>(at(off_ptrs)),
+                           reinterpret_cast<double2 *const *>(at(off_ptrs)) + n);
+        FK_HIP(hipGetLastError());
+        if (ms) {
+            FK_HIP(hipEventSynchronize(e1));
+            FK_HIP(hipEventElapsedTime(ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        return FK_OK;
+    }
+};
+
 }  // namespace
 
 int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
@@ -774,6 +1159,104 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
         }
     }
     if (prev_dev >= 0) (void)hipSetDevice(prev_dev);
+    return rc;
+}
+
+struct fk_gpp_batch {
+    fk::GppBatch b;
+};
+
+int gpp_batch_create(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,
+                     fk_gpp_batch **batch)
+{
+    clear_error();
+    if (!batch) { set_error("gpp_batch_create: NULL result pointer"); return FK_ERR_ARG; }
+    *batch = nullptr;
+    if (n < 1 || !problems || !options) { set_error("gpp_batch_create: need n >= 1, problems and options"); return FK_ERR_ARG; }
+    if (options->memory != FK_MEM_DEVICE) {
+        set_error("gpp_batch_create: batches take device memory (FK_MEM_DEVICE); call gpp_kernel per problem for host memory");
+        return FK_ERR_UNSUPPORTED;
+    }
+    if (options->precision != FK_PREC_FP64 && options->precision != FK_PREC_MIXED) {
+        set_error("gpp_batch_create: bad precision");
+        return FK_ERR_ARG;
+    }
+    if (options->config < 0 || options->config > kNumCfgs) {
+        set_error("gpp_batch_create: config %d out of range", options->config);
+        return FK_ERR_ARG;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!problems[i]) { set_error("gpp_batch_create: problem %d is NULL", i); return FK_ERR_ARG; }
+        fk_options oi = *options;
+        oi.config = 0;
+        const int rc = check_problem(problems[i], &oi);
+        if (rc != FK_OK) return rc;
+        std::vector<int32_t> cols;
+        FK_TRY(resolve_columns(problems[i]->ncouls, problems[i]->ngpown, problems[i]->inv_igp_index,
+                               problems[i]->indinv, cols));
+    }
+    fk_gpp_batch *h = new fk_gpp_batch;
+    GppBatch &b = h->b;
+    b.n = n;
+    b.mixed = options->precision == FK_PREC_MIXED;
+    b.opt = *options;
+    for (int i = 0; i < n; ++i) b.src.push_back(*problems[i]);
+    b.prob.resize(n);
+    int rc = FK_OK;
+    if (hipGetDevice(&b.device) != hipSuccess ||
+        hipDeviceGetAttribute(&b.ncu, hipDeviceAttributeMultiprocessorCount, b.device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b.cs, hipStreamNonBlocking) != hipSuccess) {
+        set_error("gpp_batch_create: device query or stream creation failed");
+        rc = FK_ERR_HIP;
+    }
+    if (rc == FK_OK) rc = b.mixed ? b.build<float2>(options->config) : b.build<double2>(options->config);
+    if (rc != FK_OK) {
+        delete h;
+        return rc;
+    }
+    *batch = h;
+    return FK_OK;
+}
+
+int gpp_batch_run(fk_gpp_batch *batch, fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, void *stream,
+                  float *elapsed_ms)
+{
+    clear_error();
+    if (!batch || !achtemp || !asxtemp) { set_error("gpp_batch_run: NULL argument"); return FK_ERR_ARG; }
+    for (int i = 0; i < batch->b.n; ++i)
+        if (!achtemp[i] || !asxtemp[i]) { set_error("gpp_batch_run: NULL output of problem %d", i); return FK_ERR_ARG; }
+    return batch->b.run(achtemp, asxtemp, static_cast<hipStream_t>(stream), elapsed_ms);
+}
+
+int gpp_batch_config(const fk_gpp_batch *batch)
+{
+    return batch ? batch->b.cfg + 1 : 0;
+}
+
+size_t gpp_batch_workspace_bytes(const fk_gpp_batch *batch)
+{
+    return batch ? batch->b.bytes : 0;
+}
+
+void gpp_batch_destroy(fk_gpp_batch *batch)
+{
+    delete batch;
+}
+
+int gpp_kernel_batched(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,
+                       fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp)
+{
+    fk_options o = options ? *options : fk_default_options();
+    if (o.config == 0) o.config = 2;   // one call: no autotuning, the single-problem default
+    fk_gpp_batch *b = nullptr;
+    FK_TRY(gpp_batch_create(n, problems, &o, &b));
+    hipStream_t s = static_cast<hipStream_t>(o.stream);
+    int rc = gpp_batch_run(b, achtemp, asxtemp, s, o.kernel_ms);
+    if (rc == FK_OK && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("gpp_kernel_batched: stream synchronisation failed");
+        rc = FK_ERR_HIP;
+    }
+    gpp_batch_destroy(b);
     return rc;
 }
 

@@ -129,6 +129,41 @@ int gpp_kernel(const fk_gpp_problem *problem, const fk_options *options,
 size_t gpp_kernel_workspace_size(const fk_gpp_problem *problem, const fk_options *options);
 
 /* ------------------------------------------------------------------------
+ * Batched GPP: many independent problems in one launch.  No counterpart in
+ * the reference, whose program runs one problem's band loop
+ * (BGW-Kernels/gppKer.cpp:303-439) per process; the batch computes exactly
+ * what gpp_kernel computes for each problem.
+ *
+ * gpp_batch_create lays out one device workspace for all problems, builds
+ * a tile table over every problem's (my_igp, ig) tiles (largest band count
+ * first), picks the tile configuration (options->config > 0, or, with 0,
+ * the fastest of all configurations timed on this batch -- create then
+ * runs the batch 2 x 4 times, so the inputs must be valid), and captures
+ * the launches of one run into a graph: one coefficient kernel, one
+ * persistent kernel whose workgroups take tiles from a shared counter, one
+ * reduction.  Inputs are device memory (FK_MEM_DEVICE) and are read again
+ * on every run, so new values in the same arrays need no new batch.
+ * Problems: nfreq <= 16; the arrays must outlive the batch.
+ * gpp_batch_run launches the graph on `stream` and writes achtemp[i],
+ * asxtemp[i] (device pointers, [nfreq_i] each); elapsed_ms (optional)
+ * receives the graph's duration and synchronises.  Calls that share one
+ * batch must be serialised by the caller.
+ * gpp_kernel_batched = create (config 2 unless set) + run + synchronise + destroy.
+ * ------------------------------------------------------------------------ */
+typedef struct fk_gpp_batch fk_gpp_batch;
+
+int gpp_batch_create(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,
+                     fk_gpp_batch **batch);
+int gpp_batch_run(fk_gpp_batch *batch, fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, void *stream,
+                  float *elapsed_ms);
+/* Tile configuration the batch uses (1-based, as fk_options.config). */
+int gpp_batch_config(const fk_gpp_batch *batch);
+size_t gpp_batch_workspace_bytes(const fk_gpp_batch *batch);
+void gpp_batch_destroy(fk_gpp_batch *batch);
+int gpp_kernel_batched(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,
+                       fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp);
+
+/* ------------------------------------------------------------------------
  * FF contraction:
  *   schDt[n1][f] = sum_my_igp conj(aqsm[n1][igp]) sum_ig I_eps[f][my_igp][ig] aqsn[n1][ig]
  *   schDv[n1][f] = the same with each column weighted by vcoul[igp]

@@ -126,3 +126,22 @@ def test_ff_workspace_size(fk):
     small = fk.ff_workspace_size(6, 40, 5, 3)
     big = fk.ff_workspace_size(256, 4096, 4096, 32)
     assert 0 < small < big < 64 << 20   # cf + column weights + per-tile partials only
+
+
+def test_batch_rejects_host_memory_and_bad_sizes(fk, fko):
+    """Argument checks of gpp_batch_create run before any device work."""
+    import ctypes
+    p = fko.gpp_random(4, 2, 64, 5, seed=1)
+    wx = np.ascontiguousarray(p.wx)
+    s = fk.GppProblem(p.nb, p.nv, p.ncouls, p.ngpown, len(wx), wx.ctypes.data, 1, 1, 1, 1, 1,
+                      p.inv_igp_index.ctypes.data, p.indinv.ctypes.data, 1e-6, 0.25, 2.5e5, 1.0)
+    arr = (ctypes.POINTER(fk.GppProblem) * 1)(ctypes.pointer(s))
+    h = ctypes.c_void_p()
+    opt = fk.default_options()          # host memory
+    assert fk.lib.gpp_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_UNSUPPORTED
+    assert "FK_MEM_DEVICE" in fk.last_error()
+    opt.memory = fk.FK_MEM_DEVICE
+    assert fk.lib.gpp_batch_create(0, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
+    opt.config = 9
+    assert fk.lib.gpp_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
+    assert not h.value
