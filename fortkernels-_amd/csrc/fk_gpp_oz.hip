@@ -260,8 +260,7 @@ constexpr int kRotFirst = 9;
 
 struct GemmArgs {
     const int8_t *as, *bs;
-    const double *ra, *rb;
-    double2 *bmat;           // [ldm][ldn] B in FP64
+    double2 *bmat;           // [ldm][ldn] B / (ra[m] rb[ig]) in FP64
     int ldm, ldn, nkh, tiles_m, tiles_n, group_m;
 };
 
@@ -430,13 +429,10 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
     half(yes{}, no{}, h0, h1, stage(s), 1, 0);
     half(no{}, no{}, h1, h0, stage(s), 0, 0);
 
-    // B = ra rb sum_L 254^-L acc_L in FP64 (int32 -> FP64 is exact)
+    // B = sum_L 254^-L acc_L in FP64 (int32 -> FP64 is exact); the row and
+    // column scales are applied where B is used
     const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0, 1.0 / 4162314256.0};
     const int col = tn * kBN + wn * 32 + (lane & 31);
-    const double rbc = a.rb[col];
-    double rar[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) rar[j] = a.ra[tm * kBM + wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3)];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const int row = tm * kBM + wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
@@ -446,8 +442,7 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
             bre += W[L] * (double)acc[L][0][j];
             bim += W[L] * (double)acc[L][1][j];
         }
-        const double sc = rar[j] * rbc;
-        a.bmat[(size_t)row * a.ldn + col] = make_double2(bre * sc, bim * sc);
+        a.bmat[(size_t)row * a.ldn + col] = make_double2(bre, bim);
     }
 }
 
@@ -458,7 +453,8 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
 // grid-stride order, so the sums are reproducible.
 // ---------------------------------------------------------------------------
 struct PoleArgs {
-    const double2 *bmat;
+    const double2 *bmat;     // [ldm][ldn] B / (ra[m] rb[ig])
+    const double *ra, *rb;   // row and column scales of the slices
     const double2 *c_occ;    // [nv][ldm]
     const float2 *aqsn, *I_eps, *wtilde;
     const int *igp;
@@ -483,7 +479,9 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
         for (int64_t e = (int64_t)blockIdx.x * blockDim.x + tid; e < n; e += stride) {
             const int m = (int)(e / a.C), ig = (int)(e - (int64_t)m * a.C);
             const bool ok = a.igp[m] >= 0;
-            const double2 B = a.bmat[(size_t)m * a.ldn + ig];
+            const double2 Bu = a.bmat[(size_t)m * a.ldn + ig];
+            const double sc = a.ra[m] * a.rb[ig];
+            const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
             double2 Bo = make_double2(0.0, 0.0);
             for (int n1 = 0; n1 < a.nv; ++n1)
                 Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
@@ -582,7 +580,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     FK_HIP(hipGetLastError());
 
     GemmArgs ga;
-    ga.as = sa.as; ga.bs = sb.bs; ga.ra = sa.ra; ga.rb = sb.rb;
+    ga.as = sa.as; ga.bs = sb.bs;
     ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
     ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = nkh;
     ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
@@ -593,7 +591,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     FK_HIP(hipGetLastError());
 
     PoleArgs pa;
-    pa.bmat = ga.bmat; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
+    pa.bmat = ga.bmat; pa.ra = sa.ra; pa.rb = sb.rb; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
     pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
