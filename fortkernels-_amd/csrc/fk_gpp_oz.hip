@@ -589,6 +589,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     FK_TRY(timer.start());
     hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(tiles), dim3(256), 0, s, ga);
     FK_HIP(hipGetLastError());
+    FK_TRY(timer.stop());
 
     PoleArgs pa;
     pa.bmat = ga.bmat; pa.ra = sa.ra; pa.rb = sb.rb; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
@@ -602,7 +603,6 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     default: hipLaunchKernelGGL(gpp_oz_pole_kernel<4>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
     }
     FK_HIP(hipGetLastError());
-    FK_TRY(timer.stop());
     FK_TRY(gpp_reduce_launch(pa.partials, kPoleBlocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
     return timer.finish();
 }

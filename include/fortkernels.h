@@ -60,9 +60,12 @@ typedef enum {
 
 typedef enum {
     FK_PREC_FP64 = 0,   /* complex128 inputs, FP64 arithmetic throughout */
-    FK_PREC_MIXED = 1   /* complex64 inputs (fk_cfloat): band contraction in
-                           FP32 with FP32 accumulation, pole factors and
-                           every cross-band / cross-tile sum in FP64 */
+    FK_PREC_MIXED = 1   /* complex64 inputs (fk_cfloat).  config 0: the band
+                           contraction on int8 MFMA by an exact-accumulation
+                           split (relative error ~1e-9 against FP64 on the
+                           same values); configs 1..4: FP32 MFMA with FP32
+                           accumulation (~1e-6).  Pole and SX factors and
+                           every cross-band / cross-tile sum are FP64. */
 } fk_precision;
 
 typedef struct {
@@ -79,8 +82,9 @@ typedef struct {
     size_t workspace_bytes;
     int32_t config;          /* kernel tile configuration; 0 = choose by shape */
     float *kernel_ms;        /* if non-NULL: the call times its dominant kernel
-                                with HIP events, synchronises, and stores the
-                                elapsed milliseconds here */
+                                (the fused GEMM kernel; the int8 GEMM kernel of
+                                the mixed path) with HIP events, synchronises,
+                                and stores the elapsed milliseconds here */
 } fk_options;
 
 /* Zero-initialised options with the canonical defaults (mask on, FP64,
