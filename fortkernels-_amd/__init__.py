@@ -298,8 +298,11 @@ class GppBatch:
 # FF
 # --------------------------------------------------------------------------
 def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index, indinv, *, achs_freq=0,
-              stream=None, workspace=None, config=0, time_kernel=False, want_schDv=True, device=-1):
-    """Run ff_kernel.  Returns (schDt[nb][nfreq], schDv or None, achsDtemp, kernel_ms|None)."""
+              stream=None, workspace=None, config=0, time_kernel=False, want_schDv=True, device=-1,
+              precision="fp64"):
+    """Run ff_kernel.  Returns (schDt[nb][nfreq], schDv or None, achsDtemp, kernel_ms|None).
+    precision "mixed" takes the same complex128 inputs and runs the contraction
+    in split BF16 on MFMA with FP32 accumulation and FP64 refinement."""
     inv = np.ascontiguousarray(inv_igp_index, np.int32)
     ind = np.ascontiguousarray(indinv, np.int32)
     on_dev = _is_torch(aqsn)
@@ -307,6 +310,7 @@ def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index
                      _ptr(vcoul), inv.ctypes.data, ind.ctypes.data, int(achs_freq))
     opt = default_options()
     opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
     opt.device = int(device)
     opt.stream = _stream_handle(stream)
     if workspace is not None:
@@ -328,12 +332,13 @@ def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index
     return schDt, schDv, achs, (float(ms.value) if time_kernel else None)
 
 
-def ff_workspace_size(nb, ncouls, ngpown, nfreq, config=0):
+def ff_workspace_size(nb, ncouls, ngpown, nfreq, config=0, precision="fp64"):
     dummy = 1  # non-NULL placeholders: only sizes are read
     inv = np.zeros(ngpown, np.int32)
     ind = np.zeros(ncouls + 1, np.int32)
     prob = FfProblem(nb, ncouls, ngpown, nfreq, dummy, dummy, dummy, dummy, inv.ctypes.data, ind.ctypes.data, 0)
     opt = default_options()
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
     opt.config = config
     return int(lib.ff_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
 

@@ -291,17 +291,28 @@ void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
 
 struct FfLayout {
     int cfg, tiles_m, mtiles, nks;
+    bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)
+    FfSplitLayout sp;
     size_t off_cf, off_vg, off_igp, off_part;
 };
 
 FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
 {
     FfLayout L;
+    L.split = o->precision == FK_PREC_MIXED;
     L.cfg = o->config > 0 ? o->config - 1 : kFfDefaultCfg;
-    const FfCfg &c = kFfCfgs[L.cfg];
-    L.tiles_m = div_up(std::max(p->number_bands, 1), c.bm);
-    L.mtiles = div_up(p->ngpown, c.bn);
-    L.nks = div_up(p->ncouls, 4 * c.p) * c.p;
+    if (L.split) {
+        ff_split_plan(p->number_bands, p->ncouls, p->ngpown, p->nfreq, o->config, ws, L.sp);
+        L.cfg = 0;
+        L.tiles_m = L.sp.nmt;
+        L.mtiles = L.sp.nkc * L.sp.ngt;
+        L.nks = L.sp.nstages;
+    } else {
+        const FfCfg &c = kFfCfgs[L.cfg];
+        L.tiles_m = div_up(std::max(p->number_bands, 1), c.bm);
+        L.mtiles = div_up(p->ngpown, c.bn);
+        L.nks = div_up(p->ncouls, 4 * c.p) * c.p;
+    }
     L.off_cf = ws.plan(sizeof(double2) * (size_t)std::max(p->number_bands, 1) * p->ngpown);
     L.off_vg = ws.plan(sizeof(double) * (size_t)p->ngpown);
     L.off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
@@ -330,9 +341,40 @@ int ff_check(const fk_ff_problem *p, const fk_options *o)
         return FK_ERR_ARG;
     }
     if (o->memory != FK_MEM_HOST && o->memory != FK_MEM_DEVICE) { set_error("ff_kernel: bad memory kind"); return FK_ERR_ARG; }
-    if (o->precision != FK_PREC_FP64) { set_error("ff_kernel: only FK_PREC_FP64 in this build"); return FK_ERR_UNSUPPORTED; }
-    if (o->config < 0 || o->config > kNumFfCfgs) { set_error("ff_kernel: config %d out of range", o->config); return FK_ERR_ARG; }
+    if (o->precision != FK_PREC_FP64 && o->precision != FK_PREC_MIXED) {
+        set_error("ff_kernel: precision must be FK_PREC_FP64 or FK_PREC_MIXED");
+        return FK_ERR_ARG;
+    }
+    const int ncfg = o->precision == FK_PREC_MIXED ? ff_split_configs() : kNumFfCfgs;
+    if (o->config < 0 || o->config > ncfg) { set_error("ff_kernel: config %d out of range [0, %d]", o->config, ncfg); return FK_ERR_ARG; }
     return FK_OK;
+}
+
+// The FP64 MFMA GEMM with its row contraction -> partials.
+int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X, const double2 *cf, const double *vg,
+                 double2 *part, int nb, int C, int G, int nf, KernelTimer &timer, hipStream_t s)
+{
+    FfArgs a;
+    a.aqsn = aqsn; a.X = X; a.cf = cf; a.vg = vg; a.partials = part;
+    a.nb = nb; a.C = C; a.G = G; a.nfreq = nf; a.nks = L.nks;
+    a.tiles_m = L.tiles_m; a.mtiles = L.mtiles; a.tiles_n = nf * L.mtiles;
+    const int tiles = a.tiles_m * a.tiles_n;
+    a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
+    if (kFfCfgs[L.cfg].minw == 2) {
+        int dev = 0, ncu = 0;
+        FK_HIP(hipGetDevice(&dev));
+        FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (tiles > 2 * ncu) {
+            const double tile_cycles = 8.0 * kFfCfgs[L.cfg].bm * kFfCfgs[L.cfg].bn * 4.0 * L.nks / 64.0;
+            a.stagger_lo = ncu;
+            a.stagger_hi = 2 * ncu;
+            a.stagger_ticks = (int)std::min(0.5 * tile_cycles / 24.0, 1.0e6);
+        }
+    }
+    FK_TRY(timer.start());
+    launch_ff_fused(L.cfg, tiles, s, a);
+    FK_HIP(hipGetLastError());
+    return timer.stop();
 }
 
 int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
@@ -351,28 +393,11 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
         hipLaunchKernelGGL(ff_coef_kernel, dim3(blocks), dim3(256), 0, s, aqsm, vcoul, igp, nb, C, G, cf, vg);
         FK_HIP(hipGetLastError());
     }
-    FfArgs a;
-    a.aqsn = aqsn; a.X = X; a.cf = cf; a.vg = vg; a.partials = part;
-    a.nb = nb; a.C = C; a.G = G; a.nfreq = nf; a.nks = L.nks;
-    a.tiles_m = L.tiles_m; a.mtiles = L.mtiles; a.tiles_n = nf * L.mtiles;
-    const int tiles = a.tiles_m * a.tiles_n;
-    a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
-    if (kFfCfgs[L.cfg].minw == 2) {
-        int dev = 0, ncu = 0;
-        FK_HIP(hipGetDevice(&dev));
-        FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        if (tiles > 2 * ncu) {
-            const double tile_cycles = 8.0 * kFfCfgs[L.cfg].bm * kFfCfgs[L.cfg].bn * 4.0 * L.nks / 64.0;
-            a.stagger_lo = ncu;
-            a.stagger_hi = 2 * ncu;
-            a.stagger_ticks = (int)std::min(0.5 * tile_cycles / 24.0, 1.0e6);
-        }
-    }
     KernelTimer timer(o->kernel_ms, s);
-    FK_TRY(timer.start());
-    launch_ff_fused(L.cfg, tiles, s, a);
-    FK_HIP(hipGetLastError());
-    FK_TRY(timer.stop());
+    if (L.split)
+        FK_TRY(ff_split_launch(L.sp, ws, aqsn, X, cf, vg, part, nb, C, G, nf, timer, s));
+    else
+        FK_TRY(ff_fp64_gemm(L, aqsn, X, cf, vg, part, nb, C, G, nf, timer, s));
     hipLaunchKernelGGL(ff_reduce_kernel, dim3(div_up(nb * nf, 256)), dim3(256), 0, s, part, nb, nf, L.mtiles, vscale,
                        schDt, schDv);
     FK_HIP(hipGetLastError());

@@ -107,4 +107,18 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
                   const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
                   double2 *d_ach, double2 *d_asx, hipStream_t s);
 
+// FF contraction in split BF16 on MFMA (FK_PREC_MIXED): workspace layout and
+// the launch of the slice pre-pass and the GEMM.  The GEMM writes FP64
+// partials [nf][nkc*ngt][nb][2] (schDt part, schDv part) for the reduction
+// of fk_ff.hip; `timer` brackets the GEMM kernel.
+struct FfSplitLayout {
+    int nstages, ns, nkc, RF, nmt, ngt, total;
+    size_t off_as;
+};
+int ff_split_configs();
+void ff_split_plan(int nb, int C, int G, int nf, int config, Scratch &ws, FfSplitLayout &L);
+int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, const double2 *X, const double2 *cf,
+                    const double *vg, double2 *partials, int nb, int C, int G, int nf, KernelTimer &timer,
+                    hipStream_t s);
+
 }  // namespace fk
