@@ -37,6 +37,7 @@ METRIC = "GPP updates/s (n1·igp·ig·iw) + % FP64/FP32 vector peak, FP64 vs mix
 PEAK_FP64 = 78.6      # TFLOP/s, MI355X FP64 vector = FP64 MFMA (dense)
 PEAK_FP32 = 157.3     # TFLOP/s, FP32 vector = FP32 MFMA
 PEAK_INT8 = 5000.0    # TOP/s, int8 MFMA (dense)
+PEAK_BF16 = 2500.0    # TFLOP/s, BF16 MFMA (dense)
 NB, NV, NCOULS, NPG, NW = 512, 2, 32768, 20, 3
 STREAMS = {"aqsm": 1, "aqsn": 2, "I_eps": 3, "wtilde": 4, "vcoul": 5}
 
@@ -179,12 +180,55 @@ def run_ff(fk, torch, steps, warmup, seed=4):
     ms = 1e3 * wall / steps
     log(f"ff: {ms:.3f} ms/step, kernel {kernel_ms:.3f} ms, {achieved:.1f} TFLOP/s "
         f"({100 * achieved / PEAK_FP64:.1f}% of FP64 peak)")
-    return {"workload": f"FF contraction {nb}/{C}/{G}/{nf} (number_bands/ncouls/ngpown/nFreq), FP64, I_epsR",
-            "value": nb * G * C * nf / (ms * 1e-3), "unit": "complex MACs/s", "ms_per_step": ms,
-            "roofline": {"bound": "mfma", "kernel": "ff_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
-                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                         "traffic": load_traffic("ff_fused_fp64_traffic.json"),
-                         "alg_flops_per_launch": flops, "kernel_ms": kernel_ms}}
+    ref = step()
+    out = {"workload": f"FF contraction {nb}/{C}/{G}/{nf} (number_bands/ncouls/ngpown/nFreq), FP64, I_epsR",
+           "value": nb * G * C * nf / (ms * 1e-3), "unit": "complex MACs/s", "ms_per_step": ms,
+           "roofline": {"bound": "mfma", "kernel": "ff_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
+                        "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                        "traffic": load_traffic("ff_fused_fp64_traffic.json"),
+                        "alg_flops_per_launch": flops, "kernel_ms": kernel_ms}}
+
+    # the same contraction recomputed in split BF16 (three slices, six slice
+    # products on MFMA, FP32 accumulation per ig chunk, FP64 refinement)
+    nbytes = fk.ff_workspace_size(nb, C, G, nf, precision="mixed")
+    ws_mix = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+    def step_mixed(time_kernel=False):
+        return fk.ff_kernel(nb, C, G, nf, aqsm, aqsn, I, v, inv, indinv, stream=stream, workspace=ws_mix,
+                            time_kernel=time_kernel, precision="mixed")
+
+    for _ in range(warmup):
+        step_mixed()
+    torch.cuda.synchronize()
+    log(f"ff split BF16: timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_mixed()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [step_mixed(time_kernel=True)[3] for _ in range(max(steps, 3))]
+    mix = step_mixed()
+    torch.cuda.synchronize()
+    rel = lambda a, b: float((a - b).abs().max() / b.abs().max())  # noqa: E731
+    err_t, err_v = rel(mix[0], ref[0]), rel(mix[1], ref[1])
+    err_a = float(abs(complex(mix[2].cpu()[0]) - complex(ref[2].cpu()[0])) / abs(complex(ref[2].cpu()[0])))
+    mms = 1e3 * wall / steps
+    mkernel = float(np.mean(kms))
+    bf16_flops = 6.0 * flops            # six slice products, each a full complex GEMM
+    mach = bf16_flops / (mkernel * 1e-3) / 1e12
+    log(f"ff split BF16: {mms:.3f} ms/step ({ms / mms:.2f}x FP64), kernel {mkernel:.3f} ms, "
+        f"{mach:.1f} TFLOP/s BF16 ({100 * mach / PEAK_BF16:.1f}% of BF16 MFMA peak), "
+        f"error vs FP64 schDt {err_t:.2e} schDv {err_v:.2e} achsDtemp {err_a:.2e}")
+    out["split_bf16"] = {
+        "ms_per_step": mms, "speedup_vs_fp64": ms / mms, "value": nb * G * C * nf / (mms * 1e-3),
+        "unit": "complex MACs/s",
+        "error_vs_fp64": {"schDt": err_t, "schDv": err_v, "achsDtemp": err_a, "norm": "max|diff|/max|ref|"},
+        "roofline": {"bound": "mfma", "kernel": "ff_split_gemm_kernel", "achieved": mach, "peak": PEAK_BF16,
+                     "unit": "TFLOP/s", "frac": mach / PEAK_BF16,
+                     "traffic": load_traffic("ff_split_gemm_traffic.json"),
+                     "alg_flops_per_launch": bf16_flops, "kernel_ms": mkernel,
+                     "fp64_equivalent_tflops": flops / (mkernel * 1e-3) / 1e12}}
+    return out
 
 
 def batched_shapes(count=64, seed=3):

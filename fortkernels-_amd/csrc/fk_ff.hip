@@ -226,35 +226,37 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
     }
 }
 
-// schDt[n1][f], schDv[n1][f]: sum over the column tiles in order
+// schDt[n1][f], schDv[n1][f]: sum over the column tiles in order.  Thread
+// (f, n1), n1 fastest, so a warp's partials are contiguous; the row sums of
+// frequency achs_freq (unscaled schDv) also go to achs_rows[n1].
 __global__ void ff_reduce_kernel(const double2 *partials, int nb, int nfreq, int mtiles, double vscale,
-                                 double2 *schDt, double2 *schDv)
+                                 double2 *schDt, double2 *schDv, int achs_freq, double2 *achs_rows)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nb * nfreq) return;
-    const int n1 = idx / nfreq, f = idx % nfreq;
+    const int f = idx / nb, n1 = idx % nb;
     double2 st = make_double2(0.0, 0.0), sv = st;
+    const double2 *p = partials + ((size_t)f * mtiles * nb + n1) * 2;
+    const size_t step = (size_t)nb * 2;
+#pragma unroll 8
     for (int mt = 0; mt < mtiles; ++mt) {
-        const double2 *p = partials + (((size_t)f * mtiles + mt) * nb + n1) * 2;
-        st = cadd(st, p[0]);
-        sv = cadd(sv, p[1]);
+        st = cadd(st, p[mt * step]);
+        sv = cadd(sv, p[mt * step + 1]);
     }
-    if (schDt) schDt[idx] = st;
-    if (schDv) schDv[idx] = make_double2(vscale * sv.x, vscale * sv.y);
+    const int o = n1 * nfreq + f;
+    if (schDt) schDt[o] = st;
+    if (schDv) schDv[o] = make_double2(vscale * sv.x, vscale * sv.y);
+    if (achs_rows && f == achs_freq) achs_rows[n1] = sv;
 }
 
 // achsDtemp = -0.5 sum_n1 schDv[n1][achs_freq], one workgroup, fixed order
-__global__ __launch_bounds__(256) void ff_achs_kernel(const double2 *partials, int nb, int nfreq, int mtiles,
-                                                       int achs_freq, double2 *achs)
+__global__ __launch_bounds__(256) void ff_achs_kernel(const double2 *achs_rows, int nb, double2 *achs)
 {
     __shared__ double sre[256], sim[256];
     double re = 0.0, im = 0.0;
     for (int n1 = threadIdx.x; n1 < nb; n1 += 256) {
-        for (int mt = 0; mt < mtiles; ++mt) {
-            const double2 v = partials[(((size_t)achs_freq * mtiles + mt) * nb + n1) * 2 + 1];
-            re += v.x;
-            im += v.y;
-        }
+        re += achs_rows[n1].x;
+        im += achs_rows[n1].y;
     }
     sre[threadIdx.x] = re;
     sim[threadIdx.x] = im;
@@ -293,7 +295,7 @@ struct FfLayout {
     int cfg, tiles_m, mtiles, nks;
     bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)
     FfSplitLayout sp;
-    size_t off_cf, off_vg, off_igp, off_part;
+    size_t off_cf, off_vg, off_igp, off_part, off_achs;
 };
 
 FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
@@ -317,6 +319,7 @@ FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
     L.off_vg = ws.plan(sizeof(double) * (size_t)p->ngpown);
     L.off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * L.mtiles * std::max(p->number_bands, 1));
+    L.off_achs = ws.plan(sizeof(double2) * (size_t)std::max(p->number_bands, 1));
     return L;
 }
 
@@ -398,11 +401,12 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
         FK_TRY(ff_split_launch(L.sp, ws, aqsn, X, cf, vg, part, nb, C, G, nf, timer, s));
     else
         FK_TRY(ff_fp64_gemm(L, aqsn, X, cf, vg, part, nb, C, G, nf, timer, s));
+    double2 *achs_rows = achs ? static_cast<double2 *>(ws.at(L.off_achs)) : nullptr;
     hipLaunchKernelGGL(ff_reduce_kernel, dim3(div_up(nb * nf, 256)), dim3(256), 0, s, part, nb, nf, L.mtiles, vscale,
-                       schDt, schDv);
+                       schDt, schDv, p->achs_freq, achs_rows);
     FK_HIP(hipGetLastError());
     if (achs) {
-        hipLaunchKernelGGL(ff_achs_kernel, dim3(1), dim3(256), 0, s, part, nb, nf, L.mtiles, p->achs_freq, achs);
+        hipLaunchKernelGGL(ff_achs_kernel, dim3(1), dim3(256), 0, s, achs_rows, nb, achs);
         FK_HIP(hipGetLastError());
     }
     return timer.finish();

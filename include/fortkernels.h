@@ -60,12 +60,19 @@ typedef enum {
 
 typedef enum {
     FK_PREC_FP64 = 0,   /* complex128 inputs, FP64 arithmetic throughout */
-    FK_PREC_MIXED = 1   /* complex64 inputs (fk_cfloat).  config 0: the band
-                           contraction on int8 MFMA by an exact-accumulation
-                           split (relative error ~1e-9 against FP64 on the
-                           same values); configs 1..4: FP32 MFMA with FP32
-                           accumulation (~1e-6).  Pole and SX factors and
-                           every cross-band / cross-tile sum are FP64. */
+    FK_PREC_MIXED = 1   /* gpp_kernel: complex64 inputs (fk_cfloat).
+                           config 0: the band contraction on int8 MFMA by an
+                           exact-accumulation split (relative error ~1e-9
+                           against FP64 on the same values); configs 1..4:
+                           FP32 MFMA with FP32 accumulation (~1e-6).  Pole
+                           and SX factors and every cross-band / cross-tile
+                           sum are FP64.
+                           ff_kernel: the same complex128 inputs; the
+                           contraction runs in split BF16 (three slices per
+                           real, six slice products on BF16 MFMA, FP32
+                           accumulation over an ig chunk) and is contracted
+                           and summed in FP64 (relative error ~5e-7 against
+                           FP64); its configs are listed at ff_kernel. */
 } fk_precision;
 
 typedef struct {
@@ -83,8 +90,10 @@ typedef struct {
     int32_t config;          /* kernel tile configuration; 0 = choose by shape */
     float *kernel_ms;        /* if non-NULL: the call times its dominant kernel
                                 (the fused GEMM kernel; the int8 GEMM kernel of
-                                the mixed path) with HIP events, synchronises,
-                                and stores the elapsed milliseconds here */
+                                the GPP mixed path; the split-BF16 GEMM kernel
+                                of the FF mixed path) with HIP events,
+                                synchronises, and stores the elapsed
+                                milliseconds here */
 } fk_options;
 
 /* Zero-initialised options with the canonical defaults (mask on, FP64,
@@ -192,7 +201,11 @@ typedef struct {
 
 /* Replaces the zgemm + contraction of ffkernel.deslippe.best.f90.back1:402-418
  * and achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:107).  schDt: [nb][nfreq];
- * schDv: [nb][nfreq] or NULL; achsDtemp: one value or NULL. */
+ * schDv: [nb][nfreq] or NULL; achsDtemp: one value or NULL.
+ * options->config: FK_PREC_FP64: 0 = default, 1 = 64x64 tiles, 2 = 64x128
+ * tiles; FK_PREC_MIXED (split BF16): ig chunk of 0 = 1024 (default),
+ * 1 = 256, 2 = 512, 3 = 1024 (a smaller chunk: less FP32 rounding, more
+ * workgroups and partials). */
 int ff_kernel(const fk_ff_problem *problem, const fk_options *options,
               fk_cdouble *schDt, fk_cdouble *schDv, fk_cdouble *achsDtemp);
 

@@ -130,13 +130,14 @@ def test_ff_workspace_size(fk):
 
 def test_ff_mixed_workspace_and_config_range(fk, fko):
     """The split-BF16 path adds aqsn's slices (6 bytes per real) and per-chunk
-    partials; its configs are 0..3 (ig chunk of 512, 256, 512, 1024)."""
+    partials; its configs are 0..3 (ig chunk of 1024, 256, 512, 1024)."""
     nb, C, G, nf = 256, 4096, 4096, 32
     mixed = fk.ff_workspace_size(nb, C, G, nf, precision="mixed")
     slices = (2 * C // 16) * (nb // 32) * 3 * 1024     # K'/16 k-steps x 32-row fragments x 3 slices x 1 KiB
-    partials = nf * (C // 512) * (G // 64) * nb * 2 * 16
+    partials = nf * (C // 1024) * (G // 64) * nb * 2 * 16
     assert mixed >= fk.ff_workspace_size(nb, C, G, nf) - 1 and mixed >= slices + partials
-    assert fk.ff_workspace_size(nb, C, G, nf, config=3, precision="mixed") < mixed
+    assert fk.ff_workspace_size(nb, C, G, nf, config=3, precision="mixed") == mixed
+    assert fk.ff_workspace_size(nb, C, G, nf, config=1, precision="mixed") > mixed
     p = fko.ff_random(4, 32, 8, 2, seed=1)
     with pytest.raises(fk.FkError) as e:
         fk.ff_problem(p, precision="mixed", config=4)

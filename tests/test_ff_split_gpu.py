@@ -13,7 +13,7 @@ What pins it:
     This pins the cross-slice products and the residual signs;
   * random inputs ~U(-0.5, 0.5): the error against the FP64 oracle is stated
     and bounded, MIXED_RTOL (normwise, max |diff| / max |ref|); measured on
-    MI355X 1.7e-7 .. 5.7e-7 for schDt and schDv, up to 1.6e-6 for achsDtemp.
+    MI355X 1.7e-7 .. 7e-7 for schDt and schDv, up to 1.6e-6 for achsDtemp.
 """
 import numpy as np
 import pytest
@@ -99,9 +99,10 @@ def test_error_vs_fp64_oracle(fk, fko, shape):
     assert abs(ga - ra) <= MIXED_RTOL * abs(ra)
 
 
-@pytest.mark.parametrize("config", [1, 3])
+@pytest.mark.parametrize("config", [1, 2])
 def test_chunk_configs(fk, fko, config):
-    """ig chunks of 256 and 1024: the chunk partials sum to the same result."""
+    """ig chunks of 256 and 512 (the default is 1024): the chunk partials sum
+    to the same result."""
     p = fko.ff_random(40, 1500, 70, 2, seed=config)
     rt, _, _ = fko.ff(p)
     gt, _, _, _ = _run(fk, p, config=config)
