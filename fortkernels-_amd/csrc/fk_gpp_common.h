@@ -13,8 +13,7 @@ constexpr int kXcds = 8;       // MI355X: consecutive workgroups go to different
 
 struct PoleConst {
     double wx[kMaxFreq];
-    double wx2[kMaxFreq];      // cden = pow(wxt,2) of the mask branch (:41)
-    double rden1[kMaxFreq];    // 1 / |cden|^2 (:42-43)
+    double iwx2[kMaxFreq];     // 1 / wx^2: conj(cden) / |cden|^2 of the mask branch, cden = wx^2 real (:41-44)
 };
 
 // Scalars of the per-element epilogue.
@@ -36,10 +35,8 @@ inline PoleParams make_pole_params(const double *wx, int nfreq, double limittwo,
     pp.nw = nfreq;
     for (int iw = 0; iw < kMaxFreq; ++iw) {
         const double w = iw < nfreq ? wx[iw] : 0.0;
-        const double w2 = w * w;
         pp.pc.wx[iw] = w;
-        pp.pc.wx2[iw] = w2;
-        pp.pc.rden1[iw] = 1.0 / (w2 * w2);
+        pp.pc.iwx2[iw] = iw < nfreq ? 1.0 / (w * w) : 0.0;
     }
     return pp;
 }
@@ -47,15 +44,20 @@ inline PoleParams make_pole_params(const double *wx, int nfreq, double limittwo,
 __device__ __forceinline__ double2 widen(double2 v) { return v; }
 __device__ __forceinline__ double2 widen(float2 v) { return make_double2((double)v.x, (double)v.y); }
 
-// 1/x from v_rcp_f64 refined by two Newton steps (within an ulp or two of
-// the IEEE quotient).
+// 1/x from v_rcp_f64 refined by STEPS Newton steps.  Measured on MI355X
+// over 4M inputs spanning 2^-40 .. 2^40: v_rcp_f64 alone is within 4.2e8
+// units of 2^-53 (about 2^-24 relative), one step within 20, two steps
+// equal the IEEE quotient.
+template <int STEPS = 2>
 __device__ __forceinline__ double rcp_fast(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
+#pragma unroll
+    for (int k = 0; k < STEPS; ++k) {
+        const double e = __builtin_fma(-x, r, 1.0);
+        r = __builtin_fma(r, e, r);
+    }
+    return r;
 }
 
 // delw = wtilde * conj(wdiff) * rden and |delw|^2 in the reference's FP64
@@ -75,16 +77,6 @@ __device__ __forceinline__ double pole_wdiffr(double wdr, double wdi)
     return wdr * wdr + wdi * wdi;
 }
 
-// ssx of the near-pole branch (:46-53): -Omega2 conj(cden)/|cden|^2 delw,
-// cden = 4 wtilde^2 (delw + 0.5).
-__device__ __forceinline__ double2 ssx_branch2(double2 wt2, double2 Om, double2 delw)
-{
-    double2 cden = cmul(make_double2(4.0 * wt2.x, 4.0 * wt2.y), make_double2(delw.x + 0.5, delw.y));
-    double rd = 1.0 / (cden.x * cden.x + cden.y * cden.y);
-    double2 t = cmul(make_double2(-Om.x, -Om.y), cconj(cden));
-    return cmul(make_double2(t.x * rd, t.y * rd), delw);
-}
-
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -94,16 +86,35 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // One (my_igp, ig) element for the NWC frequencies w0 .. w0+NWC-1: adds
 // B * D to sach[q] and Bo * ssx to sasx[q], where D = mask * delw * I_eps
-// and ssx is the three-way SX factor.  `ok` = a real, kept element; a
-// frequency past nw repeats the last one and adds nothing.
-template <int NWC>
+// and ssx is the three-way SX factor (:37-57).  `ok` = a real, kept
+// element; a frequency past nw repeats the last one and adds nothing.
+// The SX factor of the mask branch (:41-44) is Omega2 / wx^2 (cden = wx^2
+// is real); that of the near-pole branch (:49-52) is
+// -I_eps delw conj(z) / (4 |z|^2), z = delw + 0.5 (Omega2 conj(cden) /
+// |cden|^2 with cden = 4 wtilde^2 z and Omega2 = wtilde^2 I_eps loses its
+// wtilde^2).
+// HOIST = false evaluates the products per frequency, as the MFMA kernels'
+// epilogues need (fewer live registers); HOIST = true forms B I_eps,
+// Bo Omega2 and Bo I_eps once per element, for the stand-alone pole kernel:
+//   B D = (B I_eps) delw,  Bo ssx = (Bo Omega2) / wx^2  or
+//   -(Bo I_eps) delw conj(z) / (4 |z|^2).
+// Both equal the reference's expressions up to rounding.
+template <int NWC, bool HOIST = false>
 __device__ __forceinline__ void pole_element(const PoleParams &pp, int w0, bool ok, double2 wt, double2 ie,
                                              double2 B, double2 Bo, bool want_ssx, double2 (&sach)[NWC],
                                              double2 (&sasx)[NWC])
 {
+    // HOIST (the mixed-precision pole kernel): one Newton step, 2e-15
+    // relative, far inside the 1e-13 band that takes the IEEE quotient
+    constexpr int kSteps = HOIST ? 1 : 2;
     const double tol1 = 1e-13 * pp.limitone, tol2 = 1e-13 * pp.to1;
-    const double2 wt2 = cmul(wt, wt);
-    const double2 Om = cmul(wt2, ie);
+    const double2 Om = cmul(cmul(wt, wt), ie);
+    double2 Bie = B, BoOm = Om, Boie = ie;
+    if constexpr (HOIST) {
+        Bie = cmul(B, ie);
+        BoOm = cmul(Bo, Om);
+        Boie = cmul(Bo, ie);
+    }
 #pragma unroll
     for (int q = 0; q < NWC; ++q) {
         const int iw = min(w0 + q, pp.nw - 1);
@@ -112,7 +123,7 @@ __device__ __forceinline__ void pole_element(const PoleParams &pp, int w0, bool 
         const double wdiffr = pole_wdiffr(wdr, -wt.y);
         double2 delw;
         double delwr;
-        pole_delw(wt, wdr, rcp_fast(wdiffr), delw, delwr);
+        pole_delw(wt, wdr, rcp_fast<kSteps>(wdiffr), delw, delwr);
         // points within 1e-13 of a branch threshold take the IEEE quotient,
         // so every branch decision is the reference's
         const bool near = fabs(delwr - pp.limitone) <= tol1 || fabs(delwr - pp.to1) <= tol2;
@@ -120,19 +131,31 @@ __device__ __forceinline__ void pole_element(const PoleParams &pp, int w0, bool 
             if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
         }
         const bool m = (wdiffr > pp.limittwo) && (delwr < pp.limitone);
-        const double2 D = (m || !pp.mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
-        const double2 t = cmul(B, D);
-        if (live) sach[q] = cadd(sach[q], t);
+        if constexpr (HOIST) {
+            const double2 t = (live && (m || !pp.mask_on)) ? cmul(Bie, delw) : make_double2(0.0, 0.0);
+            sach[q] = cadd(sach[q], t);
+        } else {
+            const double2 D = (m || !pp.mask_on) ? cmul(delw, ie) : make_double2(0.0, 0.0);
+            const double2 t = cmul(B, D);
+            if (live) sach[q] = cadd(sach[q], t);
+        }
         if (want_ssx) {
-            // mask branch (:41-44): Omega2 * conj(wx^2) / |wx^2|^2
-            double2 S = m ? make_double2(Om.x * pp.pc.wx2[iw] * pp.pc.rden1[iw], Om.y * pp.pc.wx2[iw] * pp.pc.rden1[iw])
-                          : make_double2(0.0, 0.0);
+            double2 S = m ? make_double2(BoOm.x * pp.pc.iwx2[iw], BoOm.y * pp.pc.iwx2[iw]) : make_double2(0.0, 0.0);
             const bool need2 = !m && delwr > pp.to1;
             if (__ballot(need2)) {
-                if (need2) S = ssx_branch2(wt2, Om, delw);
+                if (need2) {
+                    const double2 z = make_double2(delw.x + 0.5, delw.y);
+                    const double sc = -0.25 * rcp_fast<kSteps>(z.x * z.x + z.y * z.y);
+                    const double2 v = cmul(Boie, cmul(delw, cconj(z)));
+                    S = make_double2(sc * v.x, sc * v.y);
+                }
             }
-            const double2 u = cmul(Bo, S);
-            if (live) sasx[q] = cadd(sasx[q], u);
+            if constexpr (HOIST) {
+                if (live) sasx[q] = cadd(sasx[q], S);
+            } else {
+                const double2 u = cmul(Bo, S);
+                if (live) sasx[q] = cadd(sasx[q], u);
+            }
         }
     }
 }

@@ -448,9 +448,11 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
 
 // ---------------------------------------------------------------------------
 // Pole/SX epilogue over every (my_igp, ig): B from bmat, Bo = sum over the
-// occupied bands in FP64, then the FP64 pole and SX factors; one partial
-// per workgroup and frequency.  Elements are dealt to threads in a fixed
-// grid-stride order, so the sums are reproducible.
+// occupied bands in FP64, then the FP64 pole and SX factors.  Workgroup b
+// takes row my_igp = b / chunks and the b % chunks-th of its columns, so
+// the row's constants (igp, its scale, the occupied coefficients) are read
+// once and the element loads are 32-bit offsets from row bases; one
+// partial per workgroup and frequency, in a fixed order.
 // ---------------------------------------------------------------------------
 struct PoleArgs {
     const double2 *bmat;     // [ldm][ldn] B / (ra[m] rb[ig])
@@ -458,8 +460,8 @@ struct PoleArgs {
     const double2 *c_occ;    // [nv][ldm]
     const float2 *aqsn, *I_eps, *wtilde;
     const int *igp;
-    double2 *partials;       // [blocks][nw][2]
-    int nv, C, G, ldm, ldn;
+    double2 *partials;       // [G * chunks][nw][2]
+    int nv, C, G, ldm, ldn, chunks;
     PoleParams pp;
 };
 
@@ -470,22 +472,27 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
     __shared__ double red[NWAVES][4 * NWC];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool want_ssx = a.nv > 0;
-    const int64_t n = (int64_t)a.G * a.C;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int m = blockIdx.x / a.chunks, ch = blockIdx.x % a.chunks;
+    const int c_lo = (int)((int64_t)ch * a.C / a.chunks), c_hi = (int)((int64_t)(ch + 1) * a.C / a.chunks);
+    const bool ok = a.igp[m] >= 0;
+    const double ram = a.ra[m];
+    const double2 *brow = a.bmat + (size_t)m * a.ldn;
+    const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
     for (int w0 = 0; w0 < a.pp.nw; w0 += NWC) {
         double2 sach[NWC], sasx[NWC];
 #pragma unroll
         for (int q = 0; q < NWC; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
-        for (int64_t e = (int64_t)blockIdx.x * blockDim.x + tid; e < n; e += stride) {
-            const int m = (int)(e / a.C), ig = (int)(e - (int64_t)m * a.C);
-            const bool ok = a.igp[m] >= 0;
-            const double2 Bu = a.bmat[(size_t)m * a.ldn + ig];
-            const double sc = a.ra[m] * a.rb[ig];
-            const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
-            double2 Bo = make_double2(0.0, 0.0);
-            for (int n1 = 0; n1 < a.nv; ++n1)
-                Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
-            pole_element<NWC>(a.pp, w0, ok, widen(a.wtilde[e]), widen(a.I_eps[e]), B, Bo, want_ssx, sach, sasx);
+        if (ok) {
+            for (int ig = c_lo + tid; ig < c_hi; ig += 256) {
+                const double2 Bu = brow[ig];
+                const double sc = ram * a.rb[ig];
+                const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
+                double2 Bo = make_double2(0.0, 0.0);
+                for (int n1 = 0; n1 < a.nv; ++n1)
+                    Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
+                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), widen(erow[ig]), B, Bo, want_ssx, sach,
+                                        sasx);
+            }
         }
 #pragma unroll
         for (int q = 0; q < NWC; ++q) {
@@ -514,7 +521,7 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
     }
 }
 
-constexpr int kPoleBlocks = 2048;
+constexpr int kPoleChunks = 4;   // column chunks per row of the pole kernel
 
 }  // namespace
 
@@ -534,7 +541,7 @@ void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
     L.off_pmax = ws.plan(sizeof(double) * (size_t)L.chunks * (L.ldm + L.ldn));
     L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
     L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
-    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * kPoleBlocks);
+    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * kPoleChunks);
 }
 
 int gpp_oz_check(const fk_gpp_problem *p)
@@ -595,15 +602,17 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     pa.bmat = ga.bmat; pa.ra = sa.ra; pa.rb = sb.rb; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
+    pa.chunks = kPoleChunks;
     pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
+    const int blocks = p->ngpown * kPoleChunks;
     switch (std::min(p->nfreq, 4)) {
-    case 1: hipLaunchKernelGGL(gpp_oz_pole_kernel<1>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
-    case 2: hipLaunchKernelGGL(gpp_oz_pole_kernel<2>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
-    case 3: hipLaunchKernelGGL(gpp_oz_pole_kernel<3>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
-    default: hipLaunchKernelGGL(gpp_oz_pole_kernel<4>, dim3(kPoleBlocks), dim3(256), 0, s, pa); break;
+    case 1: hipLaunchKernelGGL(gpp_oz_pole_kernel<1>, dim3(blocks), dim3(256), 0, s, pa); break;
+    case 2: hipLaunchKernelGGL(gpp_oz_pole_kernel<2>, dim3(blocks), dim3(256), 0, s, pa); break;
+    case 3: hipLaunchKernelGGL(gpp_oz_pole_kernel<3>, dim3(blocks), dim3(256), 0, s, pa); break;
+    default: hipLaunchKernelGGL(gpp_oz_pole_kernel<4>, dim3(blocks), dim3(256), 0, s, pa); break;
     }
     FK_HIP(hipGetLastError());
-    FK_TRY(gpp_reduce_launch(pa.partials, kPoleBlocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
+    FK_TRY(gpp_reduce_launch(pa.partials, blocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
     return timer.finish();
 }
 
