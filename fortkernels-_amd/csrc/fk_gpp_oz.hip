@@ -19,10 +19,9 @@
 // exactly.  The error against the FP64 evaluation on the same complex64
 // values is the slice truncation's, a few 1e-10 relative.
 //
-// Kernels (one call = six launches on the caller's stream):
-//   oz_max_kernel        partial row / column maxima over 64-band chunks
-//   oz_split_a_kernel    coefficients -> int8 slices, FP64 occupied rows
-//   oz_split_b_kernel    aqsn -> int8 slices
+// Kernels (one call = four launches on the caller's stream):
+//   oz_prep_kernel       row / column maxima of the coefficients and aqsn,
+//                        their int8 slices, FP64 occupied rows
 //   gpp_oz_gemm_kernel   int8 MFMA tile, operands staged through LDS by
 //                        direct global->LDS loads in a five-deep ring;
 //                        B rebuilt in FP64 and stored
@@ -54,7 +53,6 @@ constexpr int kLevels = 5;      // slice pairs with i + j < kLevels
 constexpr int kKStep = 64;      // K' per pipeline step = 32 bands = two MFMA k-halves
 constexpr int kKHalf = 32;      // K' per 32x32x32 int8 MFMA
 constexpr int kBandsPerStep = kKStep / 2;
-constexpr int kMaxChunk = 64;   // bands per partial maximum
 constexpr int kPlanesA = kSlices, kPlanesB = kSlices;
 constexpr int kFrag = 1024;     // bytes per lane-ordered fragment
 constexpr int kMaxBandsOz = 16000;   // keeps the level-3 int32 sums (4 pairs) below 2^31
@@ -84,142 +82,95 @@ __device__ __forceinline__ double2 coef(const float2 *aqsm, const double *vcoul,
 }
 
 // ---------------------------------------------------------------------------
-// Partial maxima of max(|re|, |im|) over 64-band chunks (blockIdx.y): rows
-// of the coefficients (the first ldm/64 workgroups of x) and columns of
-// aqsn (the rest).  pmax_a[chunk][ldm], pmax_b[chunk][ldn].
+// Slices of both operands in one launch.  A workgroup takes 4 coefficient
+// rows (the first ldm/4 workgroups) or 16 aqsn columns (the rest): it finds
+// their max(|re|, |im|) over all bands, then splits them band block by band
+// block (8 bands = the 16 bytes of one fragment lane); the second pass
+// finds the workgroup's slab in L2.  Thread t of an R-wide slab: row or
+// column R x + t%R, band blocks t/R, t/R + 256/R, ...  Coefficient slices
+// hold (cr, -ci) (the GEMM derives (ci, cr)); aqsn slices hold (ar, ai).
 // ---------------------------------------------------------------------------
-struct MaxArgs {
+struct PrepArgs {
     const float2 *aqsm, *aqsn;
     const double *vcoul;
     const int *igp;
-    double *pmax_a, *pmax_b;
-    int nb, C, G, ldm, ldn, chunks, blocks_a;
+    int8_t *as, *bs;         // [4][ldm/32][Kp/32][1024], [4][ldn/32][Kp/32][1024]
+    double *ra, *rb;         // M / 127 of each row, column
+    double2 *c_occ;          // [nv][ldm] coefficients of the occupied bands
+    int nb, nv, C, G, ldm, ldn, nkh, blocks_a;
 };
 
-__global__ __launch_bounds__(256) void oz_max_kernel(MaxArgs a)
+// R rows (coefficients, IS_A) or columns (aqsn) per workgroup, 256 / R band
+// groups: the coefficients are gathered at scattered igp columns, so their
+// slabs are kept thin to spread the gathers over more workgroups.
+template <int R, bool IS_A>
+__device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
 {
-    __shared__ double sm[4][64];
-    const int tid = threadIdx.x, cl = tid & 63, grp = tid >> 6;
-    const int chunk = blockIdx.y;
-    const int n_lo = chunk * kMaxChunk, n_hi = min(n_lo + kMaxChunk, a.nb);
-    double mx = 0.0;
-    const bool is_a = (int)blockIdx.x < a.blocks_a;
-    const int idx = (is_a ? (int)blockIdx.x : (int)blockIdx.x - a.blocks_a) * 64 + cl;
-    if (is_a) {
-        const int g = idx < a.G ? a.igp[idx] : -1;
-        if (g >= 0)
-            for (int n1 = n_lo + grp; n1 < n_hi; n1 += 4) {
-                const double2 c = coef(a.aqsm, a.vcoul, a.C, n1, g);
-                mx = fmax(mx, fmax(fabs(c.x), fabs(c.y)));
-            }
-    } else if (idx < a.C) {
-        for (int n1 = n_lo + grp; n1 < n_hi; n1 += 4) {
+    constexpr int NG = 256 / R;
+    __shared__ double sm[NG][R];
+    const int tid = threadIdx.x, c = tid % R, grp = tid / R;
+    const int idx = slab * R + c;
+    const int g = IS_A ? (idx < a.G ? a.igp[idx] : -1) : 0;
+    const bool in = IS_A ? g >= 0 : idx < a.C;
+    // element n1 of this row / column as (first, second) of its stored pair
+    auto value = [&](int n1) -> double2 {
+        if (!in || n1 >= a.nb) return make_double2(0.0, 0.0);
+        if constexpr (IS_A) {
+            const double2 cf = coef(a.aqsm, a.vcoul, a.C, n1, g);
+            return make_double2(cf.x, -cf.y);
+        } else {
             const float2 v = a.aqsn[(size_t)n1 * a.C + idx];
-            mx = fmax(mx, (double)fmaxf(fabsf(v.x), fabsf(v.y)));
+            return make_double2((double)v.x, (double)v.y);
         }
-    }
-    sm[grp][cl] = mx;
+    };
+    const int nblk = (a.nb + 7) / 8;
+    double mx = 0.0;
+    for (int j = grp; j < nblk; j += NG)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const double2 v = value(8 * j + b);
+            mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+        }
+    sm[grp][c] = mx;
     __syncthreads();
-    if (grp == 0) {
-        mx = fmax(fmax(sm[0][cl], sm[1][cl]), fmax(sm[2][cl], sm[3][cl]));
-        if (is_a) a.pmax_a[(size_t)chunk * a.ldm + idx] = mx;
-        else a.pmax_b[(size_t)chunk * a.ldn + idx] = mx;
+    mx = 0.0;
+    for (int k = 0; k < NG; ++k) mx = fmax(mx, sm[k][c]);
+    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
+    if (grp == 0) (IS_A ? a.ra : a.rb)[idx] = mx / 127.0;
+
+    int8_t *base = IS_A ? a.as : a.bs;
+    const size_t blk = idx >> 5, nblks = (size_t)(IS_A ? a.ldm : a.ldn) / 32;
+    for (int j = grp; j < a.nkh * 2; j += NG) {   // band blocks of the padded K'
+        int8_t p[kSlices][16];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int n1 = 8 * j + b;
+            const double2 v = value(n1);
+            if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = make_double2(v.x, -v.y);
+            int8_t s0[kSlices], s1[kSlices];
+            split4(v.x, scale, s0);
+            split4(v.y, scale, s1);
+#pragma unroll
+            for (int s = 0; s < kSlices; ++s) {
+                p[s][2 * b] = s0[s];
+                p[s][2 * b + 1] = s1[s];
+            }
+        }
+        const int lane = (idx & 31) + 32 * (j & 1), kh = j >> 1;
+#pragma unroll
+        for (int pl = 0; pl < kSlices; ++pl) {
+            int8_t *dst = base + ((pl * nblks + blk) * a.nkh + kh) * kFrag + 16 * lane;
+            *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
+        }
     }
 }
 
-// ---------------------------------------------------------------------------
-// Coefficient slices.  Workgroup (x, y) = 64 rows x one k-step (32 bands);
-// thread t: row m = 64 x + t%64, bands 32 y + 8 q .. +7 with q = t/64, i.e.
-// k-half 2 y + q/2 and lane L = m%32 + 32 (q%2) of the fragment
-// (slice, m/32, k-half) of every slice.  The stored pairs are (cr, -ci);
-// the GEMM derives (ci, cr) from them.
-// ---------------------------------------------------------------------------
-struct SplitAArgs {
-    const float2 *aqsm;
-    const double *vcoul;
-    const int *igp;
-    const double *pmax_a;
-    int8_t *as;              // [4][ldm/32][Kp/32][1024]
-    double *ra;              // [ldm] M / 127 of the row
-    double2 *c_occ;          // [nv][ldm]
-    int nb, nv, C, G, ldm, nkh, chunks;
-};
+constexpr int kPrepRowsA = 4, kPrepColsB = 16;
 
-__global__ __launch_bounds__(256) void oz_split_a_kernel(SplitAArgs a)
+__global__ __launch_bounds__(256) void oz_prep_kernel(PrepArgs a)
 {
-    const int tid = threadIdx.x, q = tid >> 6;
-    const int m = blockIdx.x * 64 + (tid & 63);
-    const int ks = blockIdx.y;
-    const int g = m < a.G ? a.igp[m] : -1;
-    double mx = 0.0;
-    for (int c = 0; c < a.chunks; ++c) mx = fmax(mx, a.pmax_a[(size_t)c * a.ldm + m]);
-    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
-    if (ks == 0 && q == 0) a.ra[m] = mx / 127.0;
-
-    int8_t p[kPlanesA][16];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const int n1 = ks * kBandsPerStep + 8 * q + b;
-        const double2 c = (g >= 0 && n1 < a.nb) ? coef(a.aqsm, a.vcoul, a.C, n1, g) : make_double2(0.0, 0.0);
-        if (n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + m] = c;
-        int8_t sr[kSlices], sn[kSlices];
-        split4(c.x, scale, sr);
-        split4(-c.y, scale, sn);
-#pragma unroll
-        for (int s = 0; s < kSlices; ++s) {
-            p[s][2 * b] = sr[s];
-            p[s][2 * b + 1] = sn[s];
-        }
-    }
-    const int lane = (m & 31) + 32 * (q & 1), kh = 2 * ks + (q >> 1);
-    const size_t rb = m >> 5, nrb = (size_t)a.ldm / 32;
-#pragma unroll
-    for (int pl = 0; pl < kPlanesA; ++pl) {
-        int8_t *dst = a.as + ((pl * nrb + rb) * a.nkh + kh) * kFrag + 16 * lane;
-        *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
-    }
-}
-
-// aqsn slices: the same scheme over columns; slice s holds (ar, ai).
-struct SplitBArgs {
-    const float2 *aqsn;
-    const double *pmax_b;
-    int8_t *bs;              // [4][ldn/32][Kp/32][1024]
-    double *rb;              // [ldn] M / 127 of the column
-    int nb, C, ldn, nkh, chunks;
-};
-
-__global__ __launch_bounds__(256) void oz_split_b_kernel(SplitBArgs a)
-{
-    const int tid = threadIdx.x, q = tid >> 6;
-    const int col = blockIdx.x * 64 + (tid & 63);
-    const int ks = blockIdx.y;
-    double mx = 0.0;
-    for (int c = 0; c < a.chunks; ++c) mx = fmax(mx, a.pmax_b[(size_t)c * a.ldn + col]);
-    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
-    if (ks == 0 && q == 0) a.rb[col] = mx / 127.0;
-
-    int8_t p[kPlanesB][16];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const int n1 = ks * kBandsPerStep + 8 * q + b;
-        const float2 v = (col < a.C && n1 < a.nb) ? a.aqsn[(size_t)n1 * a.C + col] : make_float2(0.f, 0.f);
-        int8_t sr[kSlices], si[kSlices];
-        split4((double)v.x, scale, sr);
-        split4((double)v.y, scale, si);
-#pragma unroll
-        for (int s = 0; s < kSlices; ++s) {
-            p[s][2 * b] = sr[s];
-            p[s][2 * b + 1] = si[s];
-        }
-    }
-    const int lane = (col & 31) + 32 * (q & 1), kh = 2 * ks + (q >> 1);
-    const size_t cb = col >> 5, ncb = (size_t)a.ldn / 32;
-#pragma unroll
-    for (int pl = 0; pl < kPlanesB; ++pl) {
-        int8_t *dst = a.bs + ((pl * ncb + cb) * a.nkh + kh) * kFrag + 16 * lane;
-        *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
-    }
+    if ((int)blockIdx.x < a.blocks_a) prep_slab<kPrepRowsA, true>(a, blockIdx.x);
+    else prep_slab<kPrepColsB, false>(a, blockIdx.x - a.blocks_a);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,12 +484,10 @@ void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
     L.Kp = div_up(2 * nb, kKStep) * kKStep;
     L.tiles_m = L.ldm / kBM;
     L.tiles_n = L.ldn / kBN;
-    L.chunks = div_up(nb, kMaxChunk);
     L.off_as = ws.plan((size_t)kPlanesA * L.ldm * L.Kp);
     L.off_bs = ws.plan((size_t)kPlanesB * L.ldn * L.Kp);
     L.off_ea = ws.plan(sizeof(double) * (size_t)L.ldm);
     L.off_eb = ws.plan(sizeof(double) * (size_t)L.ldn);
-    L.off_pmax = ws.plan(sizeof(double) * (size_t)L.chunks * (L.ldm + L.ldn));
     L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
     L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * kPoleChunks);
@@ -559,35 +508,20 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
                   double2 *d_ach, double2 *d_asx, hipStream_t s)
 {
     const int nks = L.Kp / kKStep, nkh = L.Kp / kKHalf;
-    double *pmax = static_cast<double *>(ws.at(L.off_pmax));
-    MaxArgs ma;
-    ma.aqsm = aqsm; ma.aqsn = aqsn; ma.vcoul = vcoul; ma.igp = d_igp;
-    ma.pmax_a = pmax; ma.pmax_b = pmax + (size_t)L.chunks * L.ldm;
-    ma.nb = p->number_bands; ma.C = p->ncouls; ma.G = p->ngpown; ma.ldm = L.ldm; ma.ldn = L.ldn;
-    ma.chunks = L.chunks; ma.blocks_a = L.ldm / 64;
-    hipLaunchKernelGGL(oz_max_kernel, dim3(L.ldm / 64 + L.ldn / 64, L.chunks), dim3(256), 0, s, ma);
-    FK_HIP(hipGetLastError());
-
-    SplitAArgs sa;
-    sa.aqsm = aqsm; sa.vcoul = vcoul; sa.igp = d_igp; sa.pmax_a = ma.pmax_a;
-    sa.as = static_cast<int8_t *>(ws.at(L.off_as));
-    sa.ra = static_cast<double *>(ws.at(L.off_ea));
-    sa.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
-    sa.nb = p->number_bands; sa.nv = p->nvband; sa.C = p->ncouls; sa.G = p->ngpown; sa.ldm = L.ldm;
-    sa.nkh = nkh; sa.chunks = L.chunks;
-    hipLaunchKernelGGL(oz_split_a_kernel, dim3(L.ldm / 64, nks), dim3(256), 0, s, sa);
-    FK_HIP(hipGetLastError());
-
-    SplitBArgs sb;
-    sb.aqsn = aqsn; sb.pmax_b = ma.pmax_b;
-    sb.bs = static_cast<int8_t *>(ws.at(L.off_bs));
-    sb.rb = static_cast<double *>(ws.at(L.off_eb));
-    sb.nb = p->number_bands; sb.C = p->ncouls; sb.ldn = L.ldn; sb.nkh = nkh; sb.chunks = L.chunks;
-    hipLaunchKernelGGL(oz_split_b_kernel, dim3(L.ldn / 64, nks), dim3(256), 0, s, sb);
+    PrepArgs pr;
+    pr.aqsm = aqsm; pr.aqsn = aqsn; pr.vcoul = vcoul; pr.igp = d_igp;
+    pr.as = static_cast<int8_t *>(ws.at(L.off_as));
+    pr.bs = static_cast<int8_t *>(ws.at(L.off_bs));
+    pr.ra = static_cast<double *>(ws.at(L.off_ea));
+    pr.rb = static_cast<double *>(ws.at(L.off_eb));
+    pr.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
+    pr.nb = p->number_bands; pr.nv = p->nvband; pr.C = p->ncouls; pr.G = p->ngpown;
+    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkh = nkh; pr.blocks_a = L.ldm / kPrepRowsA;
+    hipLaunchKernelGGL(oz_prep_kernel, dim3(L.ldm / kPrepRowsA + L.ldn / kPrepColsB), dim3(256), 0, s, pr);
     FK_HIP(hipGetLastError());
 
     GemmArgs ga;
-    ga.as = sa.as; ga.bs = sb.bs;
+    ga.as = pr.as; ga.bs = pr.bs;
     ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
     ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = nkh;
     ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
@@ -599,7 +533,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     FK_TRY(timer.stop());
 
     PoleArgs pa;
-    pa.bmat = ga.bmat; pa.ra = sa.ra; pa.rb = sb.rb; pa.c_occ = sa.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
+    pa.bmat = ga.bmat; pa.ra = pr.ra; pa.rb = pr.rb; pa.c_occ = pr.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
     pa.chunks = kPoleChunks;

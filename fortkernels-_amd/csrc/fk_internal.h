@@ -98,8 +98,8 @@ int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_sca
 // Mixed-precision GPP on int8 MFMA (complex64 inputs): workspace layout,
 // size limits, and the launch of its three kernels plus the reduction.
 struct OzLayout {
-    int ldm, ldn, Kp, tiles_m, tiles_n, chunks;
-    size_t off_as, off_bs, off_ea, off_eb, off_pmax, off_cocc, off_bmat, off_part;
+    int ldm, ldn, Kp, tiles_m, tiles_n;
+    size_t off_as, off_bs, off_ea, off_eb, off_cocc, off_bmat, off_part;
 };
 void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L);
 int gpp_oz_check(const fk_gpp_problem *p);
