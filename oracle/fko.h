@@ -23,6 +23,12 @@
  *   fko_ff           : closed form (constant inputs), numpy/BLAS matmul on
  *                      small shapes, golden vectors of the compiled
  *                      achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:107-131).
+ *   fko_ff_full      : variant 1 against the compiled C++ port's kernel
+ *                      functions on seeded inputs (golden vectors); variant
+ *                      0 by closed forms of frequency-linear inputs (exact
+ *                      interpolation, telescoping log sums), the
+ *                      occupied/unoccupied symmetries, and the parts it
+ *                      shares with variant 1.
  *   fko_fill_*       : known-answer values of the counter hash (tests).
  */
 #ifndef FKO_H
@@ -104,6 +110,39 @@ int fko_ff(int nb, int ncouls, int ngpown, int nfreq,
            const double *vcoul, const int32_t *inv_igp_index,
            const int32_t *indinv, int achs_freq, int nthreads,
            fko_z *schDt, fko_z *schDv, fko_z *achsDtemp);
+
+/* ---- whole FF evaluation (fko_ff_full.c) -------------------------------- */
+
+/* Inputs of the FF program (ffkernel.deslippe.best.f90:111-206): the
+ * frequency grid, its weights and broadening, band energies and the
+ * evaluation-frequency axis wx = freqevalmin - ekq[n1] + iw freqevalstep
+ * (iw 0-based).  schDt_in: [nb][nfreq] or NULL (then schdt_matrix is the
+ * I_epsR contraction; variant 1 uses zeros, as the C++ program does). */
+typedef struct {
+    int nb, nv, ncouls, ngpown, nfreq, nfreqeval;
+    const fko_z *aqsm, *aqsn;            /* [nb][ncouls] */
+    const fko_z *I_epsR, *I_epsA;        /* [nfreq][ngpown][ncouls] */
+    const double *vcoul;                 /* [ncouls] */
+    const int32_t *inv_igp_index;        /* [ngpown] */
+    const int32_t *indinv;               /* [ncouls+1] */
+    const double *ekq;                   /* [nb] */
+    const double *dFreqGrid;             /* [nfreq] */
+    const double *pref;                  /* [nfreq] */
+    const fko_z *dFreqBrd;               /* [nfreq] */
+    double pref_zb, freqevalmin, freqevalstep, occ;
+    int achs_freq;
+    const fko_z *schDt_in;
+} fko_ff_full_problem;
+
+typedef struct {
+    fko_z achsDtemp;
+    fko_z *asxDtemp, *achDtemp, *achDtemp_cor, *achDtemp_corb, *ach2Dtemp;  /* [nfreqeval] */
+    fko_z *schDt;                        /* [nb][nfreq] or NULL: the matrix used */
+} fko_ff_full_out;
+
+/* variant 0: the Fortran program (canonical); 1: the C++ port
+ * FFKer_cori/ffKernel.cpp with one thread.  Returns 0 or -1. */
+int fko_ff_full(const fko_ff_full_problem *p, int variant, int nthreads, fko_ff_full_out *o);
 
 #ifdef __cplusplus
 }

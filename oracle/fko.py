@@ -54,12 +54,27 @@ def lib() -> ctypes.CDLL:
         L.fko_resolve_igp.argtypes = [I, I, P, P, P]
         L.fko_ff.restype = I
         L.fko_ff.argtypes = [I, I, I, I] + [P] * 6 + [I, I] + [P] * 3
+        L.fko_ff_full.restype = I
+        L.fko_ff_full.argtypes = [ctypes.POINTER(_FfFullProblem), I, I, ctypes.POINTER(_FfFullOut)]
         _lib = L
     return _lib
 
 
 class _Z(ctypes.Structure):
     _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class _FfFullProblem(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in ("nb", "nv", "ncouls", "ngpown", "nfreq", "nfreqeval")]
+                + [(n, ctypes.c_void_p) for n in ("aqsm", "aqsn", "I_epsR", "I_epsA", "vcoul", "inv_igp_index",
+                                                   "indinv", "ekq", "dFreqGrid", "pref", "dFreqBrd")]
+                + [(n, ctypes.c_double) for n in ("pref_zb", "freqevalmin", "freqevalstep", "occ")]
+                + [("achs_freq", ctypes.c_int), ("schDt_in", ctypes.c_void_p)])
+
+
+class _FfFullOut(ctypes.Structure):
+    _fields_ = [("achsDtemp", _Z)] + [(n, ctypes.c_void_p) for n in (
+        "asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp", "schDt")]
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
@@ -254,3 +269,84 @@ def gpp_closed_form(nb, nv, ncouls, ngpown, wx):
     ach = n * 0.25 * (-0.25 + 1j * (0.5 * wx - 0.25)) / ((wx - 0.5) ** 2 + 0.25)
     asx = float(nv) * ngpown * ncouls * (-0.125 + 0.125j) / wx ** 2
     return ach, asx
+
+
+# --------------------------------------------------------------------------
+# whole FF evaluation (fko_ff_full.c)
+# --------------------------------------------------------------------------
+FF_OUTPUTS = ("asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp")
+
+
+@dataclass
+class FfFullProblem:
+    nb: int
+    nv: int
+    ncouls: int
+    ngpown: int
+    nfreq: int
+    nfreqeval: int
+    aqsm: np.ndarray      # [nb][C]
+    aqsn: np.ndarray      # [nb][C]
+    I_epsR: np.ndarray    # [nfreq][G][C]
+    I_epsA: np.ndarray    # [nfreq][G][C]
+    vcoul: np.ndarray     # [C]
+    inv_igp_index: np.ndarray
+    indinv: np.ndarray    # [C+1]
+    ekq: np.ndarray       # [nb]
+    dFreqGrid: np.ndarray  # [nfreq]
+    pref: np.ndarray      # [nfreq]
+    dFreqBrd: np.ndarray  # [nfreq] complex
+    pref_zb: float = 0.5 / 3.14
+    freqevalmin: float = 0.0
+    freqevalstep: float = 0.5
+    occ: float = 1.0
+    achs_freq: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def ff_program_axes(nb, nfreq):
+    """The FF program's band energies and frequency grid
+    (ffkernel.deslippe.best.f90:124-171): ekq[n1] = -10 + n1,
+    dFreqGrid[f] = 2 f, pref[f] = (dF[f+1] - dF[f]) / 3.14 (last = previous;
+    first and last halved), dFreqBrd = 0.1i."""
+    ekq = -10.0 + np.arange(nb, dtype=np.float64)
+    grid = 2.0 * np.arange(nfreq, dtype=np.float64)
+    pref = np.empty(nfreq)
+    for f in range(nfreq):
+        pref[f] = (grid[f + 1] - grid[f]) / 3.14 if f < nfreq - 1 else pref[f - 1]
+    pref[0] *= 0.5
+    pref[nfreq - 1] *= 0.5
+    brd = np.full(nfreq, 0.1j, dtype=np.complex128)
+    return ekq, grid, pref, brd
+
+
+def ff_full_random(nb, nv, ncouls, ngpown, nfreq, nfreqeval, seed=0) -> FfFullProblem:
+    """Seeded inputs with the program's axes: aqs*, I_epsR, I_epsA ~
+    U(-0.5,0.5) + iU(-0.5,0.5), vcoul ~ U(0.1,10), identity column maps."""
+    base = ff_random(nb, ncouls, ngpown, nfreq, seed=seed, which="I_epsR")
+    I_epsA = fill_complex(nfreq * ngpown * ncouls, seed, STREAM["I_epsA"], (-0.5, 0.5),
+                          (-0.5, 0.5)).reshape(nfreq, ngpown, ncouls)
+    ekq, grid, pref, brd = ff_program_axes(nb, nfreq)
+    return FfFullProblem(nb, nv, ncouls, ngpown, nfreq, nfreqeval, base.aqsm, base.aqsn, base.I_eps, I_epsA,
+                         base.vcoul, base.inv_igp_index, base.indinv, ekq, grid, pref, brd, meta={"seed": seed})
+
+
+def ff_full(p: FfFullProblem, variant=0, schDt_in=None, nthreads=0):
+    """Returns a dict: achsDtemp (complex), asxDtemp, achDtemp, achDtemp_cor,
+    achDtemp_corb, ach2Dtemp ([nfreqeval] complex), schDt ([nb][nfreq])."""
+    keep = [_c(p.aqsm, np.complex128), _c(p.aqsn, np.complex128), _c(p.I_epsR, np.complex128),
+            _c(p.I_epsA, np.complex128), _c(p.vcoul, np.float64), _c(p.inv_igp_index, np.int32),
+            _c(p.indinv, np.int32), _c(p.ekq, np.float64), _c(p.dFreqGrid, np.float64), _c(p.pref, np.float64),
+            _c(p.dFreqBrd, np.complex128)]
+    sin = None if schDt_in is None else _c(schDt_in, np.complex128)
+    pr = _FfFullProblem(p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval, *[a.ctypes.data for a in keep],
+                        p.pref_zb, p.freqevalmin, p.freqevalstep, p.occ, int(p.achs_freq),
+                        None if sin is None else sin.ctypes.data)
+    outs = {k: np.zeros(p.nfreqeval, np.complex128) for k in FF_OUTPUTS}
+    outs["schDt"] = np.zeros((p.nb, p.nfreq), np.complex128)
+    o = _FfFullOut(_Z(0, 0), *[outs[k].ctypes.data for k in FF_OUTPUTS + ("schDt",)])
+    rc = lib().fko_ff_full(ctypes.byref(pr), int(variant), int(nthreads), ctypes.byref(o))
+    if rc != 0:
+        raise ValueError("oracle rejected the FF problem")
+    outs["achsDtemp"] = complex(o.achsDtemp.re, o.achsDtemp.im)
+    return outs

@@ -15,7 +15,14 @@ Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
                    argv shapes, OMP_NUM_THREADS=1
   ff_achs.npz      achsDtemp of the reference's achsDtemp_Kernel
                    (FFKer_cori/ffKernel.cpp:107-131, frequency index 1)
-  ff_main.npz      achsDtemp printed by the reference's ffKernel program
+  ff_main.npz      achsDtemp, asxDtemp[0] and achDtemp_cor[0] printed by
+                   the reference's ffKernel program (constant inputs)
+  ff_full_<case>.npz  inputs + achsDtemp, asxDtemp, achDtemp, ach2Dtemp,
+                   achDtemp_cor, achDtemp_corb written by
+                   oracle/_ref/ref_ff_harness full, i.e. by the reference's
+                   achsDtemp_Kernel, asxDtemp_Kernel, achDtemp_Kernel and
+                   achDtemp_cor_Kernel (FFKer_cori/ffKernel.cpp:10-300) called
+                   in the order of its main, one thread
 
 Usage: python tests/golden/make_golden.py
 """
@@ -169,17 +176,69 @@ def ff_cases():
                         nfreq=p.nfreq, aqsm=p.aqsm, aqsn=p.aqsn, I_eps=p.I_eps, vcoul=p.vcoul,
                         inv_igp_index=p.inv_igp_index, indinv=p.indinv, achsDtemp=achs)
     print("ff_achs:", achs)
-    shapes = [(8, 2, 64, 32, 4, 3), (4, 2, 32, 16, 3, 2)]
-    rows = []
+    shapes = [(8, 2, 64, 32, 4, 3), (4, 2, 32, 16, 3, 2), (16, 4, 24, 8, 6, 10)]
+    rows = {"achsDtemp": [], "asxDtemp": [], "achDtemp_cor": []}
     for s in shapes:
         out = subprocess.run([os.path.join(REF_BIN, "ffKernel")] + [str(v) for v in s], check=True,
                              capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
-        m = re.search(r"achsDtemp = \( ([-\d.]+), ([-\d.]+)\)", out)
-        assert m, out
-        rows.append(complex(float(m.group(1)), float(m.group(2))))
+        for k in rows:
+            m = re.search(k + r" = \( ([-\d.]+), ([-\d.]+)\)", out)
+            assert m, out
+            rows[k].append(complex(float(m.group(1)), float(m.group(2))))
     np.savez_compressed(os.path.join(HERE, "ff_main.npz"), shapes=np.array(shapes, np.int64),
-                        achsDtemp=np.array(rows))
+                        **{k: np.array(v) for k, v in rows.items()})
     print("ff_main:", rows)
+
+
+def run_ff_full_harness(p: fko.FfFullProblem, schDt):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval], np.int32).tofile(f)
+            np.array([p.freqevalmin, p.freqevalstep, p.occ, p.pref_zb], np.float64).tofile(f)
+            for a in (p.aqsm, p.aqsn, p.I_epsR, p.I_epsA):
+                np.ascontiguousarray(a, np.complex128).tofile(f)
+            for a in (p.vcoul, p.ekq, p.dFreqGrid, p.pref):
+                np.ascontiguousarray(a, np.float64).tofile(f)
+            for a in (p.dFreqBrd, schDt):
+                np.ascontiguousarray(a, np.complex128).tofile(f)
+            np.ascontiguousarray(p.inv_igp_index, np.int32).tofile(f)
+            np.ascontiguousarray(p.indinv[: p.ncouls], np.int32).tofile(f)
+        subprocess.run([os.path.join(REF_BIN, "ref_ff_harness"), "full", fin, fout], check=True)
+        v = np.fromfile(fout, np.complex128)
+    ne = p.nfreqeval
+    out = {"achsDtemp": v[0]}
+    for i, k in enumerate(("asxDtemp", "achDtemp", "ach2Dtemp", "achDtemp_cor", "achDtemp_corb")):
+        out[k] = v[1 + i * ne: 1 + (i + 1) * ne]
+    return out
+
+
+def ff_full_cases():
+    """Seeded inputs for the C++ port's FF kernel functions: random band
+    energies (both signs of wx for occupied bands), a non-uniform grid, an
+    evaluation axis whose freqevalmin / freqevalstep the port truncates to
+    int, and a seeded schDt_matrix (the port's main never fills it)."""
+    rng = np.random.default_rng(2024)
+    cases = {}
+    p = fko.ff_full_random(14, 5, 40, 8, 6, 7, seed=41)
+    p.ekq = rng.uniform(-12.0, 12.0, p.nb)
+    p.dFreqGrid = np.cumsum(rng.uniform(0.5, 3.0, p.nfreq)) - 0.6
+    p.dFreqGrid[0] = 0.0
+    p.pref = rng.uniform(0.1, 0.6, p.nfreq)
+    p.dFreqBrd = rng.uniform(-0.05, 0.05, p.nfreq) + 1j * rng.uniform(0.05, 0.2, p.nfreq)
+    p.freqevalmin, p.freqevalstep, p.occ = 0.25, 1.5, 0.75
+    cases["seeded"] = (p, (rng.uniform(-1, 1, (p.nb, p.nfreq)) + 1j * rng.uniform(-1, 1, (p.nb, p.nfreq))) * 50)
+    q = fko.ff_full_random(12, 3, 24, 6, 5, 9, seed=42)     # the program's axes, schDt from the contraction
+    cases["program_axes"] = (q, fko.ff_full(q)["schDt"])
+    for name, (p, schDt) in cases.items():
+        out = run_ff_full_harness(p, schDt)
+        np.savez_compressed(os.path.join(HERE, f"ff_full_{name}.npz"),
+                            sizes=np.array([p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval], np.int64),
+                            scalars=np.array([p.pref_zb, p.freqevalmin, p.freqevalstep, p.occ]),
+                            aqsm=p.aqsm, aqsn=p.aqsn, I_epsR=p.I_epsR, I_epsA=p.I_epsA, vcoul=p.vcoul,
+                            inv_igp_index=p.inv_igp_index, indinv=p.indinv, ekq=p.ekq, dFreqGrid=p.dFreqGrid,
+                            pref=p.pref, dFreqBrd=p.dFreqBrd, schDt=schDt, **out)
+        print(f"ff_full_{name}:", {k: np.round(v, 4) for k, v in out.items()})
 
 
 def main():
@@ -193,6 +252,7 @@ def main():
     points()
     reference_main()
     ff_cases()
+    ff_full_cases()
 
 
 if __name__ == "__main__":
