@@ -33,7 +33,7 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "ff_kernel_workspace_size", "fk_last_error", "fk_abi_version", "fk_build_target",
            "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe",
            "gpp_batch_create", "gpp_batch_run", "gpp_batch_config", "gpp_batch_workspace_bytes",
-           "gpp_batch_destroy", "gpp_kernel_batched"]
+           "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size"]
 
 
 class FkError(RuntimeError):
@@ -66,6 +66,22 @@ class FfProblem(ctypes.Structure):
                 ("inv_igp_index", ctypes.c_void_p), ("indinv", ctypes.c_void_p), ("achs_freq", ctypes.c_int32)]
 
 
+class FfFullProblem(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int32) for n in ("number_bands", "nvband", "ncouls", "ngpown", "nfreq", "nfreqeval")]
+                + [(n, ctypes.c_void_p) for n in ("aqsmtemp", "aqsntemp", "I_epsR_array", "I_epsA_array", "vcoul",
+                                                   "inv_igp_index", "indinv", "ekq", "dFreqGrid", "pref",
+                                                   "dFreqBrd")]
+                + [(n, ctypes.c_double) for n in ("pref_zb", "freqevalmin", "freqevalstep", "occ")]
+                + [("achs_freq", ctypes.c_int32)])
+
+
+FF_FULL_OUTPUTS = ("asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp")
+
+
+class FfFullOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("achsDtemp",) + FF_FULL_OUTPUTS + ("schDt",)]
+
+
 def _load():
     # torch (when installed) ships its own HIP runtime whose SONAME is the
     # libamdhip64.so.7 this library needs.  Loading torch first makes the
@@ -92,6 +108,11 @@ def _load():
     L.ff_kernel.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options), P, P, P]
     L.ff_kernel_workspace_size.restype = SZ
     L.ff_kernel_workspace_size.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options)]
+    L.ff_kernel_full.restype = ctypes.c_int
+    L.ff_kernel_full.argtypes = [ctypes.POINTER(FfFullProblem), ctypes.POINTER(Options),
+                                 ctypes.POINTER(FfFullOutputs)]
+    L.ff_kernel_full_workspace_size.restype = SZ
+    L.ff_kernel_full_workspace_size.argtypes = [ctypes.POINTER(FfFullProblem), ctypes.POINTER(Options)]
     L.fk_last_error.restype = ctypes.c_char_p
     L.fk_last_error.argtypes = []
     L.fk_abi_version.restype = ctypes.c_int
@@ -346,6 +367,78 @@ def ff_workspace_size(nb, ncouls, ngpown, nfreq, config=0, precision="fp64"):
 def ff_problem(p, **kw):
     return ff_kernel(p.nb, p.ncouls, p.ngpown, p.nfreq, p.aqsm, p.aqsn, p.I_eps, p.vcoul, p.inv_igp_index,
                      p.indinv, **kw)
+
+
+def _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, host, scal):
+    return FfFullProblem(int(nb), int(nv), int(ncouls), int(ngpown), int(nfreq), int(nfreqeval), _ptr(aqsm),
+                         _ptr(aqsn), _ptr(I_epsR), _ptr(I_epsA), _ptr(vcoul), *[h.ctypes.data for h in host],
+                         *[float(x) for x in scal[:4]], int(scal[4]))
+
+
+def ff_kernel_full(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, inv_igp_index,
+                   indinv, ekq, dFreqGrid, pref, dFreqBrd, *, pref_zb=0.5 / 3.14, freqevalmin=0.0,
+                   freqevalstep=0.5, occ=1.0, achs_freq=0, stream=None, workspace=None, config=0,
+                   time_kernel=False, want_schDt=True, device=-1, precision="fp64"):
+    """Run ff_kernel_full (the whole FF evaluation of the Fortran program).
+    Bulk arrays are numpy (host) or torch device tensors; inv_igp_index,
+    indinv, ekq, dFreqGrid, pref, dFreqBrd are host arrays.  Returns a dict
+    with achsDtemp, asxDtemp, achDtemp, achDtemp_cor, achDtemp_corb,
+    ach2Dtemp, schDt (or None) as numpy / torch like the inputs, and
+    kernel_ms (the call's device time, or None)."""
+    host = [np.ascontiguousarray(inv_igp_index, np.int32), np.ascontiguousarray(indinv, np.int32),
+            np.ascontiguousarray(ekq, np.float64), np.ascontiguousarray(dFreqGrid, np.float64),
+            np.ascontiguousarray(pref, np.float64), np.ascontiguousarray(dFreqBrd, np.complex128)]
+    on_dev = _is_torch(aqsn)
+    prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, host,
+                           (pref_zb, freqevalmin, freqevalstep, occ, achs_freq))
+    opt = default_options()
+    opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.device = int(device)
+    opt.stream = _stream_handle(stream)
+    if workspace is not None:
+        opt.workspace = _ptr(workspace)
+        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.config = int(config)
+    ms = ctypes.c_float(-1.0)
+    if time_kernel:
+        opt.kernel_ms = ctypes.pointer(ms)
+    if on_dev:
+        import torch
+        mk = lambda *s: torch.zeros(*s, dtype=torch.complex128, device=aqsn.device)  # noqa: E731
+    else:
+        mk = lambda *s: np.zeros(s, np.complex128)  # noqa: E731
+    res = {"achsDtemp": mk(1)}
+    for k in FF_FULL_OUTPUTS:
+        res[k] = mk(nfreqeval)
+    res["schDt"] = mk(nb, nfreq) if want_schDt else None
+    outs = FfFullOutputs(*[_ptr(res[k]) for k in ("achsDtemp",) + FF_FULL_OUTPUTS + ("schDt",)])
+    _check(lib.ff_kernel_full(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(outs)))
+    res["kernel_ms"] = float(ms.value) if time_kernel else None
+    return res
+
+
+def ff_full_workspace_size(nb, nv, ncouls, ngpown, nfreq, nfreqeval, ekq, dFreqGrid, config=0, precision="fp64",
+                           freqevalmin=0.0, freqevalstep=0.5):
+    """Workspace bytes of ff_kernel_full; it depends on the evaluation axis
+    (the I_epsA bands and frequencies that are read)."""
+    dummy = np.zeros(1)
+    host = [np.zeros(ngpown, np.int32), np.zeros(ncouls + 1, np.int32), np.ascontiguousarray(ekq, np.float64),
+            np.ascontiguousarray(dFreqGrid, np.float64), np.zeros(nfreq), np.zeros(nfreq, np.complex128)]
+    prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, dummy, dummy, dummy, dummy, dummy, host,
+                           (0.0, freqevalmin, freqevalstep, 1.0, 0))
+    opt = default_options()
+    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.config = config
+    return int(lib.ff_kernel_full_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
+
+
+def ff_full_problem(p, **kw):
+    """ff_kernel_full on an object with the oracle's FfFullProblem fields."""
+    return ff_kernel_full(p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval, p.aqsm, p.aqsn, p.I_epsR,
+                          p.I_epsA, p.vcoul, p.inv_igp_index, p.indinv, p.ekq, p.dFreqGrid, p.pref, p.dFreqBrd,
+                          pref_zb=p.pref_zb, freqevalmin=p.freqevalmin, freqevalstep=p.freqevalstep, occ=p.occ,
+                          achs_freq=p.achs_freq, **kw)
 
 
 # --------------------------------------------------------------------------

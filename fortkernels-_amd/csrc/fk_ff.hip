@@ -291,12 +291,7 @@ void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
     }
 }
 
-struct FfLayout {
-    int cfg, tiles_m, mtiles, nks;
-    bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)
-    FfSplitLayout sp;
-    size_t off_cf, off_vg, off_igp, off_part, off_achs;
-};
+}  // namespace
 
 FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
 {
@@ -354,8 +349,8 @@ int ff_check(const fk_ff_problem *p, const fk_options *o)
 }
 
 // The FP64 MFMA GEMM with its row contraction -> partials.
-int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X, const double2 *cf, const double *vg,
-                 double2 *part, int nb, int C, int G, int nf, KernelTimer &timer, hipStream_t s)
+static int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X, const double2 *cf, const double *vg,
+                        double2 *part, int nb, int C, int G, int nf, KernelTimer &timer, hipStream_t s)
 {
     FfArgs a;
     a.aqsn = aqsn; a.X = X; a.cf = cf; a.vg = vg; a.partials = part;
@@ -411,8 +406,6 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
     }
     return timer.finish();
 }
-
-}  // namespace
 
 int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
                      const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,

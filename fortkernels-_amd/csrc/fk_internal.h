@@ -121,4 +121,22 @@ int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, co
                     const double *vg, double2 *partials, int nb, int C, int G, int nf, KernelTimer &timer,
                     hipStream_t s);
 
+// The FF contraction of one I_eps array (fk_ff.hip): ff_plan carves its
+// workspace out of `ws` (several plans can share one Scratch), ff_launch
+// runs the coefficient kernel, the GEMM (FP64 MFMA, or split BF16 for
+// FK_PREC_MIXED) and the fixed-order reduction into schDt / schDv (either
+// may be NULL) and, if achs is non-NULL, achsDtemp at p->achs_freq.
+// options->kernel_ms times the GEMM kernel.
+struct FfLayout {
+    int cfg, tiles_m, mtiles, nks;
+    bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)
+    FfSplitLayout sp;
+    size_t off_cf, off_vg, off_igp, off_part, off_achs;
+};
+FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws);
+int ff_check(const fk_ff_problem *p, const fk_options *o);
+int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
+              const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s);
+
 }  // namespace fk

@@ -13,6 +13,8 @@
  *                 and the static-CH sum achsDtemp_Kernel
  *                 (FFKer_cori/ffKernel.cpp:107-131; Fortran
  *                 ffkernel.deslippe.best.f90:251-267).
+ *   ff_kernel_full()  the whole FF evaluation of that Fortran program
+ *                 (static CH, SX, sigma 1-4, delta terms; :207-631).
  *
  * Conventions (the reference's functions are void and accumulate into
  * caller-zeroed outputs; these entry points instead):
@@ -210,6 +212,78 @@ int ff_kernel(const fk_ff_problem *problem, const fk_options *options,
               fk_cdouble *schDt, fk_cdouble *schDv, fk_cdouble *achsDtemp);
 
 size_t ff_kernel_workspace_size(const fk_ff_problem *problem, const fk_options *options);
+
+/* ------------------------------------------------------------------------
+ * Whole FF self-energy evaluation: the Fortran program
+ * ffkernel.deslippe.best.f90:251-631, with schdt_matrix from the contraction
+ * of ffkernel.deslippe.best.f90.back1:402-418.  For the evaluation axis
+ *   wx[n1][iw] = freqevalmin - ekq[n1] + iw * freqevalstep    (iw 0-based)
+ * and S = schDt (I_epsR contraction, no vcoul), VR / VA = the vcoul-weighted
+ * contractions of I_epsR / I_epsA (ff_kernel's schDv):
+ *   achsDtemp         -0.5 sum_n1 VR[n1][achs_freq]                  :251-267
+ *   asxDtemp[iw]      occ * sum over occupied bands of VR interpolated at
+ *                     wx (wx >= 0) or VA interpolated at -wx (wx < 0) :278-386
+ *   achDtemp[iw]      sigma 1: sum_{f<nfreq-1} -i pref[f] S[f] /
+ *                              (wx - dFreqGrid[f] + dFreqBrd[f])        :488-494
+ *   achDtemp_corb[iw] sigma 3: the same with -dFreqGrid[f] for occupied bands
+ *   ach2Dtemp[iw]     sigma 2 (log-weighted averages of neighbouring S,
+ *                     f >= 1) + delta terms (VR - VA interpolated)     :495-613
+ *   achDtemp_cor[iw]  sigma 4 + delta terms
+ * summed over bands n1 (occupied: n1 < nvband).  Interval search, its
+ * fall-back to the last interval, the 1e-4..1e4 clamps of the log ratios
+ * and the band/frequency ranges of each term follow the Fortran.  The C++
+ * port FFKer_cori/ffKernel.cpp:10-300 departs from the Fortran (evaluation
+ * axis, aliased outputs, an uncomputed schDt_matrix) and is not reproduced.
+ * I_epsA is contracted only over the bands and frequencies the SX and delta
+ * terms read.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t number_bands;        /* nb */
+    int32_t nvband;              /* occupied bands, 0..nb */
+    int32_t ncouls;              /* C */
+    int32_t ngpown;              /* G */
+    int32_t nfreq;               /* nFreq >= 2 grid frequencies */
+    int32_t nfreqeval;           /* evaluation frequencies >= 1 */
+    const void *aqsmtemp;        /* [nb][ncouls] fk_cdouble */
+    const void *aqsntemp;        /* [nb][ncouls] fk_cdouble */
+    const void *I_epsR_array;    /* [nfreq][ngpown][ncouls] fk_cdouble */
+    const void *I_epsA_array;    /* [nfreq][ngpown][ncouls] fk_cdouble */
+    const double *vcoul;         /* [ncouls] */
+    const int32_t *inv_igp_index;/* [ngpown]   HOST */
+    const int32_t *indinv;       /* [ncouls+1] HOST */
+    const double *ekq;           /* [nb]       HOST band energies */
+    const double *dFreqGrid;     /* [nfreq]    HOST, strictly increasing */
+    const double *pref;          /* [nfreq]    HOST quadrature weights */
+    const fk_cdouble *dFreqBrd;  /* [nfreq]    HOST broadening */
+    double pref_zb;              /* 0.5/3.14 in the program */
+    double freqevalmin;          /* 0 */
+    double freqevalstep;         /* 0.5 */
+    double occ;                  /* 1 */
+    int32_t achs_freq;           /* 0 as the Fortran */
+} fk_ff_full_problem;
+
+typedef struct {
+    fk_cdouble *achsDtemp;       /* one value, or NULL */
+    fk_cdouble *asxDtemp;        /* [nfreqeval] */
+    fk_cdouble *achDtemp;        /* [nfreqeval] */
+    fk_cdouble *achDtemp_cor;    /* [nfreqeval] */
+    fk_cdouble *achDtemp_corb;   /* [nfreqeval] */
+    fk_cdouble *ach2Dtemp;       /* [nfreqeval] */
+    fk_cdouble *schDt;           /* [nb][nfreq] schdt_matrix, or NULL */
+} fk_ff_full_outputs;
+
+/* Replaces the band loop of ffkernel.deslippe.best.f90:207-631 (the static
+ * CH, SX and CH terms) and the C++ port's achsDtemp_Kernel, asxDtemp_Kernel,
+ * achDtemp_Kernel and achDtemp_cor_Kernel (FFKer_cori/ffKernel.cpp:107-254).
+ * Output pointers follow options->memory.  options->precision: FK_PREC_FP64
+ * or FK_PREC_MIXED (both contractions in split BF16, as ff_kernel; the
+ * evaluation is FP64 either way); options->config as ff_kernel;
+ * options->kernel_ms: the whole call's device time (both contractions and
+ * the evaluation). */
+int ff_kernel_full(const fk_ff_full_problem *problem, const fk_options *options,
+                   const fk_ff_full_outputs *outputs);
+
+size_t ff_kernel_full_workspace_size(const fk_ff_full_problem *problem, const fk_options *options);
 
 /* ------------------------------------------------------------------------
  * Utilities

@@ -161,3 +161,28 @@ def test_batch_rejects_host_memory_and_bad_sizes(fk, fko):
     opt.config = 9
     assert fk.lib.gpp_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
     assert not h.value
+
+
+def test_ff_full_workspace_and_checks(fk, fko):
+    """ff_kernel_full's workspace covers the I_epsR contraction and, for the
+    bands and frequencies the SX / delta terms read, the I_epsA one; its
+    argument checks run before any device work."""
+    nb, C, G, nf, ne = 64, 512, 256, 16, 10
+    ekq, grid, _, _ = fko.ff_program_axes(nb, nf)
+    base = fk.ff_workspace_size(nb, C, G, nf)
+    # no occupied band and every wx < 0: I_epsA is never read
+    none = fk.ff_full_workspace_size(nb, 0, C, G, nf, ne, ekq + 100.0, grid)
+    occ = fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid)
+    assert none >= base and occ > none
+    assert fk.ff_full_workspace_size(nb, 8, C, G, 1, ne, ekq, grid[:1]) == 0          # nfreq >= 2
+    bad = grid.copy()
+    bad[3] = bad[2]
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, bad) == 0               # strictly increasing
+    assert fk.ff_full_workspace_size(nb, nb + 1, C, G, nf, ne, ekq, grid) == 0         # nv <= nb
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, 0, ekq, grid) == 0               # nfreqeval >= 1
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, precision="mixed") > 0
+    p = fko.ff_full_random(6, 2, 16, 4, 3, 4, seed=1)
+    p.dFreqGrid = p.dFreqGrid[::-1].copy()
+    with pytest.raises(fk.FkError) as e:
+        fk.ff_full_problem(p)
+    assert e.value.code == fk.FK_ERR_ARG and "increasing" in fk.last_error()
