@@ -138,6 +138,7 @@ def run_gpp(fk, torch, precision, steps, warmup, seed=1):
 
 
 FF_NB, FF_C, FF_G, FF_NF = 256, 4096, 4096, 32
+FF_NV, FF_NE = 8, 10
 
 
 def run_ff(fk, torch, steps, warmup, seed=4):
@@ -228,7 +229,64 @@ def run_ff(fk, torch, steps, warmup, seed=4):
                      "traffic": load_traffic("ff_split_gemm_traffic.json"),
                      "alg_flops_per_launch": bf16_flops, "kernel_ms": mkernel,
                      "fp64_equivalent_tflops": flops / (mkernel * 1e-3) / 1e12}}
+    del ws, ws_mix
+    out["full_evaluation"] = run_ff_full(fk, torch, steps, warmup, aqsm, aqsn, I, v, inv, indinv, seed)
     return out
+
+
+def run_ff_full(fk, torch, steps, warmup, aqsm, aqsn, IR, v, inv, indinv, seed):
+    """ff_kernel_full: the whole FF evaluation of the Fortran program (static
+    CH, SX, sigma 1-4, delta terms) at nb=256, nvband=8, 4096/4096/32,
+    nfreqeval=10 with the program's axes; FP64 and split BF16."""
+    dev = torch.device("cuda:0")
+    nb, C, G, nf = FF_NB, FF_C, FF_G, FF_NF
+    nv, ne = FF_NV, FF_NE
+    IA = torch.empty(nf, G, C, dtype=torch.complex128, device=dev)
+    fk.fill_complex(IA, seed, 7, (-0.5, 0.5), (-0.5, 0.5))
+    ekq = -10.0 + np.arange(nb, dtype=np.float64)           # ffkernel.deslippe.best.f90:124-131
+    grid = 2.0 * np.arange(nf, dtype=np.float64)
+    pref = np.empty(nf)
+    pref[:-1] = (grid[1:] - grid[:-1]) / 3.14
+    pref[-1] = pref[-2]
+    pref[0] *= 0.5
+    pref[-1] *= 0.5
+    brd = np.full(nf, 0.1j)
+    stream = torch.cuda.current_stream()
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        n = fk.ff_full_workspace_size(nb, nv, C, G, nf, ne, ekq, grid, precision=prec)
+        ws = torch.empty(n, dtype=torch.uint8, device=dev)
+
+        def step(time_kernel=False):
+            return fk.ff_kernel_full(nb, nv, C, G, nf, ne, aqsm, aqsn, IR, IA, v, inv, indinv, ekq, grid, pref,
+                                     brd, stream=stream, workspace=ws, time_kernel=time_kernel, precision=prec,
+                                     want_schDt=False)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        log(f"ff full evaluation {prec}: timing {steps} steps")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [step(time_kernel=True)["kernel_ms"] for _ in range(3)]
+        outs[prec] = step()
+        torch.cuda.synchronize()
+        res[prec] = {"ms_per_step": 1e3 * wall / steps, "device_ms": float(np.mean(kms))}
+        del ws
+    rel = lambda a, b: float((a - b).abs().max() / b.abs().max())  # noqa: E731
+    err = {k: rel(outs["mixed"][k], outs["fp64"][k]) for k in fk.FF_FULL_OUTPUTS + ("achsDtemp",)}
+    res["split_bf16_error_vs_fp64"] = dict(err, norm="max|diff|/max|ref| over the evaluation frequencies")
+    res["speedup_split_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    res["workload"] = (f"FF evaluation {nb}/{nv}/{C}/{G}/{nf}/{ne} (number_bands/nvband/ncouls/ngpown/nFreq/"
+                       "nfreqeval), I_epsR contracted in full, I_epsA over the bands and frequencies read")
+    log(f"ff full evaluation: FP64 {res['fp64']['ms_per_step']:.3f} ms/step, split BF16 "
+        f"{res['mixed']['ms_per_step']:.3f} ms/step; split error vs FP64 "
+        + " ".join(f"{k} {v:.1e}" for k, v in err.items()))
+    del IA
+    return res
 
 
 def batched_shapes(count=64, seed=3):

@@ -279,7 +279,11 @@ struct FfCfg { int bm, bn, p, minw; };
 //   0: 32x32 waves, 2x2 waves (64x64 tile), ring 3, 2 workgroups/CU
 //   1: 32x64 waves, 2x2 waves (64x128 tile), ring 3, 1 workgroup/CU (default:
 //      15.4 ms against 16.2 ms for id 0 at 256/4096/4096/32 on MI355X)
-constexpr FfCfg kFfCfgs[] = {{64, 64, 3, 2}, {64, 128, 3, 1}};
+//   2: 16x32 waves, 1x4 waves (16x128 tile), ring 3, 2 workgroups/CU: for
+//      16 bands or fewer, where a 64-row tile would spend 4x the MFMA work
+//      (kFfConfigFewBands)
+constexpr FfCfg kFfCfgs[] = {{64, 64, 3, 2}, {64, 128, 3, 1}, {16, 128, 3, 2}};
+static_assert(kFfConfigFewBands == 3, "kFfConfigFewBands names the 16-row tile");
 constexpr int kFfDefaultCfg = 1;
 constexpr int kNumFfCfgs = sizeof(kFfCfgs) / sizeof(kFfCfgs[0]);
 
@@ -287,6 +291,7 @@ void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
 {
     switch (cfg) {
     case 0: hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((ff_fused_kernel<1, 2, 3, 1, 4, 2>), dim3(tiles), dim3(256), 0, s, a); break;
     default: hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1>), dim3(tiles), dim3(256), 0, s, a); break;
     }
 }

@@ -234,6 +234,7 @@ struct FullPlan {
     int bA0, nbA, fA0, nFA;
     std::vector<double> tab;            // wx, wxp, grid, pref, brd (re, im)
     FfLayout LR, LA;
+    fk_options oA;                      // options of the I_epsA contraction
     size_t off_tab, off_S, off_VR, off_VA, off_part;
 };
 
@@ -277,6 +278,21 @@ void full_axes(const fk_ff_full_problem *p, FullPlan &P)
     P.nchunks = div_up(nb, kEpiThreads);
 }
 
+// The I_epsA contraction covers few bands at the program's axes (15 of
+// 256): up to 16 bands it runs on the FP64 GEMM with 16-row tiles in either
+// precision (exact, and 4x less MFMA work than a 64-row tile, 16x less than
+// the split path's 256-row tile); otherwise as the caller asked.
+fk_options a_options(const fk_options *o, int nbA)
+{
+    fk_options oa = *o;
+    oa.kernel_ms = nullptr;
+    if (nbA <= 16) {
+        oa.precision = FK_PREC_FP64;
+        oa.config = kFfConfigFewBands;
+    }
+    return oa;
+}
+
 void full_plan(const fk_ff_full_problem *p, const fk_options *o, Scratch &ws, FullPlan &P)
 {
     full_axes(p, P);
@@ -284,7 +300,8 @@ void full_plan(const fk_ff_full_problem *p, const fk_options *o, Scratch &ws, Fu
     P.LR = ff_plan(&pr, o, ws);
     if (P.haveA) {
         const fk_ff_problem pa = sub_problem(p, P.nbA, P.nFA, nullptr, nullptr, nullptr, 0);
-        P.LA = ff_plan(&pa, o, ws);
+        P.oA = a_options(o, P.nbA);
+        P.LA = ff_plan(&pa, &P.oA, ws);
     }
     P.off_tab = ws.plan(sizeof(double) * P.tab.size());
     P.off_S = ws.plan(sizeof(double2) * (size_t)P.nb * P.nF);
@@ -322,7 +339,7 @@ int full_launch(const fk_ff_full_problem *p, const fk_options *o, const FullPlan
         const double2 *am = io.aqsm + (size_t)P.bA0 * C, *an = io.aqsn + (size_t)P.bA0 * C;
         const double2 *X = io.IA + (io.A_is_slice ? 0 : (size_t)P.fA0 * GC);
         const fk_ff_problem pa = sub_problem(p, P.nbA, P.nFA, am, an, X, 0);
-        FK_TRY(ff_launch(&pa, &oi, P.LA, ws, cols, am, an, X, io.vcoul, nullptr, VA, nullptr, 1.0, s));
+        FK_TRY(ff_launch(&pa, &P.oA, P.LA, ws, cols, am, an, X, io.vcoul, nullptr, VA, nullptr, 1.0, s));
     }   // else no (n1, iw) reads VA
     EpiArgs a;
     a.S = S; a.VR = VR; a.VA = VA;

@@ -127,6 +127,8 @@ int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, co
 // FK_PREC_MIXED) and the fixed-order reduction into schDt / schDv (either
 // may be NULL) and, if achs is non-NULL, achsDtemp at p->achs_freq.
 // options->kernel_ms times the GEMM kernel.
+// fk_options.config of the FP64 FF GEMM with 16-row tiles (for few bands)
+constexpr int kFfConfigFewBands = 3;
 struct FfLayout {
     int cfg, tiles_m, mtiles, nks;
     bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)

@@ -71,12 +71,61 @@ def test_gppker_random_matches_library(fk, fko):
 
 
 @pytest.mark.gpu
-def test_ffker_matches_reference_program():
+def test_ffker_matches_reference_program(fko):
+    """achsDtemp as the C++ program prints it (frequency index 1); asxDtemp,
+    achDtemp_cor and the rest follow the Fortran program, checked against the
+    oracle's canonical variant on the same constant inputs."""
     d = np.load(os.path.join(GOLDEN, "ff_main.npz"))
     for (nb, nv, C, G, nf, nfe), ref in zip(d["shapes"], d["achsDtemp"]):
+        nb, nv, C, G, nf, nfe = (int(x) for x in (nb, nv, C, G, nf, nfe))
+        print(f"ffKer {nb} {nv} {C} {G} {nf} {nfe}", flush=True)
         r = _run("ffKer", nb, nv, C, G, nf, nfe)
         assert r.returncode == 0, r.stderr
         m = re.search(r"^achsDtemp \(full precision\) = \(([^,]+), ([^)]+)\)$", r.stdout, re.M)
         got = complex(float(m.group(1)), float(m.group(2)))
         assert abs(got - ref) <= 1e-13 * abs(ref)
-        assert re.search(r"^achsDtemp = \( -?\d+\.\d{6}, -?\d+\.\d{6}\) $", r.stdout, re.M)
+        for name in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+            assert re.search(rf"^{name} = \( -?\d+\.\d{{6}}, -?\d+\.\d{{6}}\) $", r.stdout, re.M), name
+        assert re.search(r"^ Answer: \(", r.stdout, re.M)
+        ekq, grid, pref, brd = fko.ff_program_axes(nb, nf)
+        z = np.full((nb, C), 0.5 + 0.5j)
+        inv = np.arange(G, dtype=np.int32)
+        ind = np.arange(C + 1, dtype=np.int32)
+        ind[C] = C - 1
+        p = fko.FfFullProblem(nb, nv, C, G, nf, nfe, z, z.copy(), np.full((nf, G, C), 0.5 + 0.5j),
+                              np.full((nf, G, C), 0.5 - 0.5j), np.ones(C), inv, ind, ekq, grid, pref, brd)
+        want = fko.ff_full(p)
+        for name in fko.FF_OUTPUTS:
+            got = np.array(_complex_lines(r.stdout, name))
+            assert got.shape == want[name].shape, name
+            assert np.max(np.abs(got - want[name])) <= 1e-12 * max(np.max(np.abs(want[name])), 1e-300), name
+
+
+@pytest.mark.gpu
+def test_ffker_random_matches_oracle(fko):
+    """--random fills aqs*, I_epsR, I_epsA, vcoul on the device with the seeded
+    generator the oracle shares; every printed output equals the oracle's."""
+    nb, nv, C, G, nf, nfe = 16, 4, 24, 8, 6, 10
+    r = _run("ffKer", nb, nv, C, G, nf, nfe, "--random", 3, "--achs-freq", 0)
+    assert r.returncode == 0, r.stderr
+    want = fko.ff_full(fko.ff_full_random(nb, nv, C, G, nf, nfe, seed=3))
+    for name in fko.FF_OUTPUTS:
+        got = np.array(_complex_lines(r.stdout, name))
+        assert np.max(np.abs(got - want[name])) <= 1e-12 * np.max(np.abs(want[name])), name
+    m = re.search(r"^achsDtemp \(full precision\) = \(([^,]+), ([^)]+)\)$", r.stdout, re.M)
+    assert abs(complex(float(m.group(1)), float(m.group(2))) - want["achsDtemp"]) <= 1e-12 * abs(want["achsDtemp"])
+    r2 = _run("ffKer", nb, nv, C, G, nf, nfe, "--random", 3, "--achs-freq", 0, "--mixed")
+    assert r2.returncode == 0, r2.stderr
+    got = np.array(_complex_lines(r2.stdout, "achDtemp"))
+    assert np.max(np.abs(got - want["achDtemp"])) <= 1e-5 * np.max(np.abs(want["achDtemp"]))
+
+
+@pytest.mark.gpu
+def test_ffker_contraction_mode(fko):
+    d = np.load(os.path.join(GOLDEN, "ff_main.npz"))
+    (nb, nv, C, G, nf, nfe), ref = d["shapes"][0], d["achsDtemp"][0]
+    r = _run("ffKer", nb, nv, C, G, nf, nfe, "--contraction", "--both")
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"^achsDtemp \(full precision\) = \(([^,]+), ([^)]+)\)$", r.stdout, re.M)
+    assert abs(complex(float(m.group(1)), float(m.group(2))) - ref) <= 1e-13 * abs(ref)
+    assert "schDt[I_epsA]" in r.stdout

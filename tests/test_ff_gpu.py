@@ -108,7 +108,7 @@ def test_achs_frequency(fk, fko, achs_freq):
     _check(fko, fk, p, achs_freq=achs_freq)
 
 
-@pytest.mark.parametrize("config", [1, 2])
+@pytest.mark.parametrize("config", [1, 2, 3])
 def test_tile_configs(fk, fko, config):
     p = fko.ff_random(100, 333, 150, 2, seed=config)
     _check(fko, fk, p, config=config)
