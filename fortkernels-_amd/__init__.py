@@ -33,7 +33,8 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "ff_kernel_workspace_size", "fk_last_error", "fk_abi_version", "fk_build_target",
            "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe",
            "gpp_batch_create", "gpp_batch_run", "gpp_batch_config", "gpp_batch_workspace_bytes",
-           "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size"]
+           "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size",
+           "gpp_kernel_ex"]
 
 
 class FkError(RuntimeError):
@@ -64,6 +65,10 @@ class FfProblem(ctypes.Structure):
                 ("nfreq", ctypes.c_int32), ("aqsmtemp", ctypes.c_void_p), ("aqsntemp", ctypes.c_void_p),
                 ("I_eps_array", ctypes.c_void_p), ("vcoul", ctypes.c_void_p),
                 ("inv_igp_index", ctypes.c_void_p), ("indinv", ctypes.c_void_p), ("achs_freq", ctypes.c_int32)]
+
+
+class GppOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("achtemp", "asxtemp", "acht_n1_loc", "achstemp")]
 
 
 class FfFullProblem(ctypes.Structure):
@@ -108,6 +113,8 @@ def _load():
     L.ff_kernel.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options), P, P, P]
     L.ff_kernel_workspace_size.restype = SZ
     L.ff_kernel_workspace_size.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options)]
+    L.gpp_kernel_ex.restype = ctypes.c_int
+    L.gpp_kernel_ex.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), ctypes.POINTER(GppOutputs)]
     L.ff_kernel_full.restype = ctypes.c_int
     L.ff_kernel_full.argtypes = [ctypes.POINTER(FfFullProblem), ctypes.POINTER(Options),
                                  ctypes.POINTER(FfFullOutputs)]
@@ -186,9 +193,12 @@ def default_options() -> Options:
 # --------------------------------------------------------------------------
 def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv_igp_index, indinv, *,
                to1=1e-6, limittwo=0.25, limitone=2.5e5, occ=1.0, precision="fp64", mask=True,
-               stream=None, workspace=None, config=0, time_kernel=False, device=-1, acht_n1=False):
+               stream=None, workspace=None, config=0, time_kernel=False, device=-1, acht_n1=False,
+               achstemp=False):
     """Run gpp_kernel.  Returns (achtemp, asxtemp, kernel_ms|None), and with
     acht_n1=True (achtemp, asxtemp, kernel_ms|None, acht_n1_loc[nb]).
+    achstemp=True (through gpp_kernel_ex) appends the static-CH remainder
+    achstemp (a one-element array / tensor) as the last item.
 
     numpy inputs -> host memory mode, outputs are numpy complex128 arrays.
     torch device tensors -> device mode, outputs are torch complex128 device
@@ -219,13 +229,21 @@ def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv
         ach = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
         asx = torch.empty(nw, dtype=torch.complex128, device=aqsn.device)
         n1 = torch.empty(max(int(nb), 1), dtype=torch.complex128, device=aqsn.device) if acht_n1 else None
+        achs = torch.empty(1, dtype=torch.complex128, device=aqsn.device) if achstemp else None
     else:
         ach = np.zeros(nw, np.complex128)
         asx = np.zeros(nw, np.complex128)
         n1 = np.zeros(max(int(nb), 1), np.complex128) if acht_n1 else None
-    _check(lib.gpp_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(ach), _ptr(asx), _ptr(n1)))
+        achs = np.zeros(1, np.complex128) if achstemp else None
+    if achstemp:
+        outs = GppOutputs(_ptr(ach), _ptr(asx), _ptr(n1), _ptr(achs))
+        _check(lib.gpp_kernel_ex(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(outs)))
+    else:
+        _check(lib.gpp_kernel(ctypes.byref(prob), ctypes.byref(opt), _ptr(ach), _ptr(asx), _ptr(n1)))
     out = (ach, asx, (float(ms.value) if time_kernel else None))
-    return out + (n1[:int(nb)],) if acht_n1 else out
+    if acht_n1:
+        out = out + (n1[:int(nb)],)
+    return out + (achs,) if achstemp else out
 
 
 def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0):

@@ -47,6 +47,7 @@ struct GppArgs {
     const E *wtilde;       // [G][C]
     const int *igp;        // [G], -1 = skipped column
     double2 *partials;     // [tiles][nw][2] (achtemp part, asxtemp part)
+    double2 *achs_part;    // [tiles] static-CH remainder part (ACHS tiles only)
     int nb, nv, C, G, nw, ldc, Kpad;
     int nvpad;             // occupied bands rounded up to a k-step (4)
     int nks;               // k-steps (Kpad / 4)
@@ -182,7 +183,10 @@ struct TileLds {
 
 // One (tm, tn) tile of one problem: GEMM, epilogue, partials.  Ends with
 // a workgroup barrier, so the LDS can be reused for the next tile.
-template <class E, int MB, int NB, int P, int WGM, int WGN, int NWC>
+// ACHS: also the static-CH remainder of the tile, -sum B I_eps over its kept
+// (my_igp, ig) (gppkernel.deslippe.09.30.2014.f90:192-230; B carries the
+// 0.5 vcoul of the coefficients), into achs_part[tile].
+template <class E, int MB, int NB, int P, int WGM, int WGN, int NWC, bool ACHS>
 __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, TileLds<E, MB, NB, WGM * WGN, NWC> &sh)
 {
     using A4 = typename Acc<E>::type;
@@ -293,6 +297,7 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
         }
 
     const size_t tile = (size_t)tm + (size_t)tn * a.tiles_m;
+    double2 sachs = make_double2(0.0, 0.0);
     for (int w0 = 0; w0 < a.nw; w0 += NWC) {
         double2 sach[NWC], sasx[NWC];
 #pragma unroll
@@ -316,6 +321,9 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
                     const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
                     const double2 Bo = widen(occ[((i * NB + j) * 4 + r) * 64 + lane]);
                     pole_element<NWC>(a.pp, w0, ok, widen(cwt[r]), widen(cie[r]), B, Bo, want_ssx, sach, sasx);
+                    if constexpr (ACHS) {
+                        if (w0 == 0 && ok) sachs = csub(sachs, cmul(B, widen(cie[r])));
+                    }
                 }
             }
         // wave reduction, then a fixed-order sum over the waves
@@ -344,10 +352,25 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
         }
         __syncthreads();
     }
+    if constexpr (ACHS) {
+        const double v0 = wave_sum(sachs.x), v1 = wave_sum(sachs.y);
+        if (lane == 0) {
+            red[wave][0] = v0;
+            red[wave][1] = v1;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            double sum = red[0][tid];
+#pragma unroll
+            for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
+            reinterpret_cast<double *>(a.achs_part + tile)[tid] = sum;
+        }
+        __syncthreads();
+    }
 }
 
 // One problem, one tile per workgroup.
-template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
+template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC, bool ACHS>
 __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs<E> a)
 {
     __shared__ TileLds<E, MB, NB, WGM * WGN, NWC> sh;
@@ -361,7 +384,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
     }
     int tm, tn;
     tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
-    gpp_tile<E, MB, NB, P, WGM, WGN, NWC>(a, tm, tn, sh);
+    gpp_tile<E, MB, NB, P, WGM, WGN, NWC, ACHS>(a, tm, tn, sh);
 }
 
 // Many problems in one launch: a fixed grid of workgroups takes tiles from a
@@ -383,14 +406,37 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_batch_kernel(const G
         __syncthreads();
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
-        gpp_tile<E, MB, NB, P, WGM, WGN, NWC>(probs[bt.prob], bt.tm, bt.tn, sh);
+        gpp_tile<E, MB, NB, P, WGM, WGN, NWC, false>(probs[bt.prob], bt.tm, bt.tn, sh);
     }
 }
 
-__global__ void zero_outputs_kernel(double2 *a, double2 *b, int n)
+__global__ void zero_outputs_kernel(double2 *a, double2 *b, int n, double2 *c)
 {
     int i = threadIdx.x;
     if (i < n) { a[i] = make_double2(0.0, 0.0); b[i] = make_double2(0.0, 0.0); }
+    if (c && i == 0) *c = make_double2(0.0, 0.0);
+}
+
+// out = sum_t part[t], one workgroup, fixed order
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double2 *part, int n, double2 *out)
+{
+    __shared__ double s_re[256], s_im[256];
+    double re = 0.0, im = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        re += part[t].x;
+        im += part[t].y;
+    }
+    s_re[threadIdx.x] = re;
+    s_im[threadIdx.x] = im;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_re[threadIdx.x] += s_re[threadIdx.x + w];
+            s_im[threadIdx.x] += s_im[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = make_double2(s_re[0], s_im[0]);
 }
 
 // One workgroup per output value: sums the per-tile partials in a fixed order.
@@ -461,27 +507,35 @@ struct TileCfg { int bm, bn, p, minw; };
 constexpr TileCfg kCfgs[] = {{32, 256, 4, 1}, {64, 64, 3, 2}, {128, 64, 3, 2}, {64, 128, 3, 2}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <class E, int NWC>
+template <class E, int NWC, bool ACHS>
 void launch_fused_nw(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 {
     switch (cfg) {
-    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 4, 1, 4, 1, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 2, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((gpp_fused_kernel<E, 4, 2, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 3, 2, 2, 2, NWC>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 0: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 4, 1, 4, 1, NWC, ACHS>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 2, 3, 2, 2, 2, NWC, ACHS>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((gpp_fused_kernel<E, 4, 2, 3, 2, 2, 2, NWC, ACHS>), dim3(tiles), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((gpp_fused_kernel<E, 2, 4, 3, 2, 2, 2, NWC, ACHS>), dim3(tiles), dim3(256), 0, s, a); break;
     }
 }
 
-// Frequencies per epilogue pass: all of them up to 4, else passes of 4.
+template <class E, bool ACHS>
+void launch_fused_achs(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
+{
+    switch (a.nw) {
+    case 1: launch_fused_nw<E, 1, ACHS>(cfg, tiles, s, a); break;
+    case 2: launch_fused_nw<E, 2, ACHS>(cfg, tiles, s, a); break;
+    case 3: launch_fused_nw<E, 3, ACHS>(cfg, tiles, s, a); break;
+    default: launch_fused_nw<E, 4, ACHS>(cfg, tiles, s, a); break;
+    }
+}
+
+// Frequencies per epilogue pass: all of them up to 4, else passes of 4; the
+// static-CH remainder only in the kernels of calls that ask for it.
 template <class E>
 void launch_fused(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 {
-    switch (a.nw) {
-    case 1: launch_fused_nw<E, 1>(cfg, tiles, s, a); break;
-    case 2: launch_fused_nw<E, 2>(cfg, tiles, s, a); break;
-    case 3: launch_fused_nw<E, 3>(cfg, tiles, s, a); break;
-    default: launch_fused_nw<E, 4>(cfg, tiles, s, a); break;
-    }
+    if (a.achs_part) launch_fused_achs<E, true>(cfg, tiles, s, a);
+    else launch_fused_achs<E, false>(cfg, tiles, s, a);
 }
 
 int choose_cfg(const fk_gpp_problem *p, const fk_options *o)
@@ -495,7 +549,7 @@ int choose_cfg(const fk_gpp_problem *p, const fk_options *o)
 
 struct Layout {
     int cfg, Kpad, nvpad, ldc, tiles_m, tiles_n;
-    size_t off_c, off_igp, off_part;
+    size_t off_c, off_igp, off_part, off_achs;
 };
 
 Layout plan_layout(const fk_gpp_problem *p, const fk_options *o, Scratch &ws, size_t elem)
@@ -512,6 +566,7 @@ Layout plan_layout(const fk_gpp_problem *p, const fk_options *o, Scratch &ws, si
     L.off_c = ws.plan((size_t)L.Kpad * L.ldc * elem);
     L.off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * L.tiles_m * L.tiles_n);
+    L.off_achs = ws.plan(sizeof(double2) * (size_t)L.tiles_m * L.tiles_n);   // used when achstemp is asked for
     return L;
 }
 
@@ -540,11 +595,11 @@ int check_problem(const fk_gpp_problem *p, const fk_options *o)
 template <class E>
 int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scratch &ws,
                const std::vector<int32_t> &cols, const E *aqsm, const E *aqsn, const E *ieps, const E *wt,
-               const double *vcoul, double2 *d_ach, double2 *d_asx, hipStream_t s)
+               const double *vcoul, double2 *d_ach, double2 *d_asx, double2 *d_achs, hipStream_t s)
 {
     const int C = p->ncouls, G = p->ngpown;
-    if (p->number_bands == 0) {   // no band: both sums are empty
-        hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, d_ach, d_asx, p->nfreq);
+    if (p->number_bands == 0) {   // no band: every sum is empty
+        hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, d_ach, d_asx, p->nfreq, d_achs);
         FK_HIP(hipGetLastError());
         return FK_OK;
     }
@@ -571,6 +626,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
 
     GppArgs<E> a;
     a.c = c; a.aqsn = aqsn; a.I_eps = ieps; a.wtilde = wt; a.igp = igp; a.partials = part;
+    a.achs_part = d_achs ? static_cast<double2 *>(ws.at(L.off_achs)) : nullptr;
     a.nb = p->number_bands; a.nv = p->nvband; a.C = C; a.G = G; a.nw = p->nfreq; a.ldc = L.ldc; a.Kpad = L.Kpad;
     a.nvpad = L.nvpad; a.nks = L.Kpad / 4;
     a.tiles_m = L.tiles_m; a.tiles_n = L.tiles_n;
@@ -601,6 +657,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     hipLaunchKernelGGL(gpp_reduce_kernel, dim3(2 * p->nfreq), dim3(256), 0, s, part, tiles, p->nfreq,
                        2.0 * p->occ, d_ach, d_asx);
     FK_HIP(hipGetLastError());
+    if (d_achs) FK_TRY(gpp_sum_launch(a.achs_part, tiles, d_achs, s));
     return timer.finish();
 }
 
@@ -645,7 +702,7 @@ int launch_acht_n1(const fk_gpp_problem *p, const fk_options *o, const std::vect
 // Host-memory path: upload, run, download (synchronous).
 template <class E, class Launch>
 int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int32_t> &cols, double2 *ach,
-             double2 *asx, double2 *n1, hipStream_t s, Launch &&launch)
+             double2 *asx, double2 *n1, double2 *achs, hipStream_t s, Launch &&launch)
 {
     const size_t nbC = (size_t)p->number_bands * p->ncouls, GC = (size_t)p->ngpown * p->ncouls;
     E *dm = nullptr, *dn = nullptr, *die = nullptr, *dwt = nullptr;
@@ -666,20 +723,24 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int
     alloc((void **)&die, GC * sizeof(E));
     alloc((void **)&dwt, GC * sizeof(E));
     alloc((void **)&dv, sizeof(double) * p->ncouls);
-    const size_t nout = 2 * (size_t)p->nfreq + (n1 ? (size_t)p->number_bands : 0);
+    const size_t nout = 2 * (size_t)p->nfreq + 1 + (n1 ? (size_t)p->number_bands : 0);
     alloc((void **)&dout, sizeof(double2) * nout);
     copy(dm, p->aqsmtemp, nbC * sizeof(E));
     copy(dn, p->aqsntemp, nbC * sizeof(E));
     copy(die, p->I_eps_array, GC * sizeof(E));
     copy(dwt, p->wtilde_array, GC * sizeof(E));
     copy(dv, p->vcoul, sizeof(double) * p->ncouls);
-    if (rc == FK_OK) rc = launch(dm, dn, die, dwt, dv, dout, dout + p->nfreq);
-    if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, dout + 2 * p->nfreq, s);
+    double2 *d_achs = achs ? dout + 2 * p->nfreq : nullptr;
+    double2 *d_n1 = dout + 2 * p->nfreq + 1;
+    if (rc == FK_OK) rc = launch(dm, dn, die, dwt, dv, dout, dout + p->nfreq, d_achs);
+    if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, d_n1, s);
     if (rc == FK_OK) {
         hipError_t e = hipMemcpyAsync(ach, dout, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipMemcpyAsync(asx, dout + p->nfreq, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess && n1 && p->number_bands > 0)
-            e = hipMemcpyAsync(n1, dout + 2 * p->nfreq, sizeof(double2) * p->number_bands, hipMemcpyDeviceToHost, s);
+            e = hipMemcpyAsync(n1, d_n1, sizeof(double2) * p->number_bands, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && achs)
+            e = hipMemcpyAsync(achs, d_achs, sizeof(double2), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { set_error("download: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
     } else {
@@ -1038,6 +1099,13 @@ struct GppBatch {
 
 }  // namespace
 
+int gpp_sum_launch(const double2 *partials, int n, double2 *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, partials, n, out);
+    FK_HIP(hipGetLastError());
+    return FK_OK;
+}
+
 int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
                       double2 *asxtemp, hipStream_t s)
 {
@@ -1072,9 +1140,20 @@ size_t gpp_kernel_workspace_size(const fk_gpp_problem *p, const fk_options *o)
 int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp, fk_cdouble *asxtemp,
                fk_cdouble *acht_n1_loc)
 {
+    fk_gpp_outputs out;
+    out.achtemp = achtemp;
+    out.asxtemp = asxtemp;
+    out.acht_n1_loc = acht_n1_loc;
+    out.achstemp = nullptr;
+    return gpp_kernel_ex(p, o, &out);
+}
+
+int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_outputs *out)
+{
     clear_error();
     FK_TRY(check_problem(p, o));
-    if (!achtemp || !asxtemp) { set_error("gpp_kernel: NULL output"); return FK_ERR_ARG; }
+    if (!out || !out->achtemp || !out->asxtemp) { set_error("gpp_kernel: NULL output"); return FK_ERR_ARG; }
+    fk_cdouble *achtemp = out->achtemp, *asxtemp = out->asxtemp, *acht_n1_loc = out->acht_n1_loc;
     if (acht_n1_loc && p->nfreq < 3) {   // the per-band sum is taken at frequency index 2 (:424)
         set_error("gpp_kernel: acht_n1_loc needs nfreq >= 3 (got %d)", p->nfreq);
         return FK_ERR_ARG;
@@ -1095,6 +1174,7 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
     if (oz) FK_TRY(gpp_oz_check(p));
     double2 *ach = reinterpret_cast<double2 *>(achtemp), *asx = reinterpret_cast<double2 *>(asxtemp);
     double2 *n1 = reinterpret_cast<double2 *>(acht_n1_loc);
+    double2 *achs = reinterpret_cast<double2 *>(out->achstemp);
     int rc;
     if (oz) {
         Scratch ws;
@@ -1108,22 +1188,22 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
             if (e != hipSuccess) { set_error("column map upload: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; }
         }
         auto launch = [&](const float2 *m, const float2 *n, const float2 *ie, const float2 *w, const double *v,
-                          double2 *da, double2 *dx) -> int {
+                          double2 *da, double2 *dx, double2 *dc) -> int {
             if (p->number_bands == 0) {
-                hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, da, dx, p->nfreq);
+                hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, da, dx, p->nfreq, dc);
                 FK_HIP(hipGetLastError());
                 return FK_OK;
             }
-            return gpp_oz_launch(p, o, Z, ws, d_igp, m, n, ie, w, v, da, dx, s);
+            return gpp_oz_launch(p, o, Z, ws, d_igp, m, n, ie, w, v, da, dx, dc, s);
         };
         if (rc == FK_OK) {
             if (o->memory == FK_MEM_DEVICE) {
                 const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
                 const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
-                rc = launch(m, n, ie, w, p->vcoul, ach, asx);
+                rc = launch(m, n, ie, w, p->vcoul, ach, asx, achs);
                 if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
             } else {
-                rc = run_host<float2>(p, o, igp, ach, asx, n1, s, launch);
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, s, launch);
             }
         }
     } else {
@@ -1135,25 +1215,27 @@ int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp
                 if (mixed) {
                     const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
                     const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
-                    rc = launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
+                    rc = launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
                     if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
                 } else {
                     const double2 *m = (const double2 *)p->aqsmtemp, *n = (const double2 *)p->aqsntemp;
                     const double2 *ie = (const double2 *)p->I_eps_array, *w = (const double2 *)p->wtilde_array;
-                    rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
+                    rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
                     if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
                 }
             } else if (mixed) {
-                rc = run_host<float2>(p, o, igp, ach, asx, n1, s,
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, s,
                                       [&](const float2 *m, const float2 *n, const float2 *ie, const float2 *w,
-                                          const double *v, double2 *da, double2 *dx) {
-                                          return launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);
+                                          const double *v, double2 *da, double2 *dx, double2 *dc) {
+                                          return launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, dc,
+                                                                    s);
                                       });
             } else {
-                rc = run_host<double2>(p, o, igp, ach, asx, n1, s,
+                rc = run_host<double2>(p, o, igp, ach, asx, n1, achs, s,
                                        [&](const double2 *m, const double2 *n, const double2 *ie, const double2 *w,
-                                           const double *v, double2 *da, double2 *dx) {
-                                           return launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);
+                                           const double *v, double2 *da, double2 *dx, double2 *dc) {
+                                           return launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, dc,
+                                                                      s);
                                        });
             }
         }

@@ -412,6 +412,7 @@ struct PoleArgs {
     const float2 *aqsn, *I_eps, *wtilde;
     const int *igp;
     double2 *partials;       // [G * chunks][nw][2]
+    double2 *achs_part;      // [G * chunks] static-CH remainder, or NULL
     int nv, C, G, ldm, ldn, chunks;
     PoleParams pp;
 };
@@ -429,6 +430,10 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
     const double ram = a.ra[m];
     const double2 *brow = a.bmat + (size_t)m * a.ldn;
     const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
+    // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:
+    // 192-230), taken with the first frequency pass when asked for
+    const bool want_achs = a.achs_part != nullptr;
+    double2 sachs = make_double2(0.0, 0.0);
     for (int w0 = 0; w0 < a.pp.nw; w0 += NWC) {
         double2 sach[NWC], sasx[NWC];
 #pragma unroll
@@ -441,8 +446,9 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
                 double2 Bo = make_double2(0.0, 0.0);
                 for (int n1 = 0; n1 < a.nv; ++n1)
                     Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
-                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), widen(erow[ig]), B, Bo, want_ssx, sach,
-                                        sasx);
+                const double2 ie = widen(erow[ig]);
+                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), ie, B, Bo, want_ssx, sach, sasx);
+                if (want_achs && w0 == 0) sachs = csub(sachs, cmul(B, ie));
             }
         }
 #pragma unroll
@@ -470,6 +476,20 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
         }
         __syncthreads();
     }
+    if (want_achs) {
+        const double v0 = wave_sum(sachs.x), v1 = wave_sum(sachs.y);
+        if (lane == 0) {
+            red[wave][0] = v0;
+            red[wave][1] = v1;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            double sum = red[0][tid];
+#pragma unroll
+            for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
+            reinterpret_cast<double *>(a.achs_part + blockIdx.x)[tid] = sum;
+        }
+    }
 }
 
 constexpr int kPoleChunks = 4;   // column chunks per row of the pole kernel
@@ -491,6 +511,7 @@ void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
     L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
     L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * kPoleChunks);
+    L.off_achs = ws.plan(sizeof(double2) * (size_t)p->ngpown * kPoleChunks);
 }
 
 int gpp_oz_check(const fk_gpp_problem *p)
@@ -505,7 +526,7 @@ int gpp_oz_check(const fk_gpp_problem *p)
 
 int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, Scratch &ws, const int *d_igp,
                   const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
-                  double2 *d_ach, double2 *d_asx, hipStream_t s)
+                  double2 *d_ach, double2 *d_asx, double2 *d_achs, hipStream_t s)
 {
     const int nks = L.Kp / kKStep, nkh = L.Kp / kKHalf;
     PrepArgs pr;
@@ -535,6 +556,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     PoleArgs pa;
     pa.bmat = ga.bmat; pa.ra = pr.ra; pa.rb = pr.rb; pa.c_occ = pr.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
+    pa.achs_part = d_achs ? static_cast<double2 *>(ws.at(L.off_achs)) : nullptr;
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
     pa.chunks = kPoleChunks;
     pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
@@ -547,6 +569,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     }
     FK_HIP(hipGetLastError());
     FK_TRY(gpp_reduce_launch(pa.partials, blocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
+    if (d_achs) FK_TRY(gpp_sum_launch(pa.achs_part, blocks, d_achs, s));
     return timer.finish();
 }
 

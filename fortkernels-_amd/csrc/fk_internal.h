@@ -92,6 +92,9 @@ int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *a
 
 // Fixed-order sum of GPP per-tile partials [tiles][nw][2] into achtemp and
 // asxtemp (asxtemp scaled by asx_scale).
+// out = the fixed-order sum of n partials (one workgroup).
+int gpp_sum_launch(const double2 *partials, int n, double2 *out, hipStream_t s);
+
 int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
                       double2 *asxtemp, hipStream_t s);
 
@@ -99,13 +102,13 @@ int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_sca
 // size limits, and the launch of its three kernels plus the reduction.
 struct OzLayout {
     int ldm, ldn, Kp, tiles_m, tiles_n;
-    size_t off_as, off_bs, off_ea, off_eb, off_cocc, off_bmat, off_part;
+    size_t off_as, off_bs, off_ea, off_eb, off_cocc, off_bmat, off_part, off_achs;
 };
 void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L);
 int gpp_oz_check(const fk_gpp_problem *p);
 int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, Scratch &ws, const int *d_igp,
                   const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
-                  double2 *d_ach, double2 *d_asx, hipStream_t s);
+                  double2 *d_ach, double2 *d_asx, double2 *d_achs, hipStream_t s);
 
 // FF contraction in split BF16 on MFMA (FK_PREC_MIXED): workspace layout and
 // the launch of the slice pre-pass and the GEMM.  The GEMM writes FP64

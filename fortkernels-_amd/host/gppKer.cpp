@@ -20,6 +20,9 @@
 //   --mixed         complex64 storage, FP32 band contraction (FK_PREC_MIXED)
 //   --unmasked      the GPU variants' unmasked sum (gppKer_gpuComplex.cpp:171-178)
 //   --acht-n1       also compute and print acht_n1_loc[n1] (:424)
+//   --achstemp      also compute and print the static-CH remainder achstemp of
+//                   the Fortran program (gppkernel.deslippe.09.30.2014.f90:
+//                   192-230; gpp_kernel_ex)
 //   --repeat K      run the kernel K times and print the mean kernel time
 //   --device D      HIP device ordinal
 #include <hip/hip_runtime_api.h>
@@ -43,7 +46,7 @@ void usage()
 {
     std::cout << "The correct form of input is : " << std::endl;
     std::cout << " ./a.out <number_bands> <number_valence_bands> <number_plane_waves> <nodes_per_mpi_group> "
-                 "[--random SEED] [--near-pole] [--nfreq N] [--mixed] [--unmasked] [--acht-n1] [--repeat K] "
+                 "[--random SEED] [--near-pole] [--nfreq N] [--mixed] [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
                  "[--device D]"
               << std::endl;
 }
@@ -64,7 +67,7 @@ void hip_check(hipError_t e, const char *what)
 
 struct Args {
     int nb = 0, nv = 0, ncouls = 0, npg = 0;
-    bool random = false, near_pole = false, mixed = false, unmasked = false, acht_n1 = false;
+    bool random = false, near_pole = false, mixed = false, unmasked = false, acht_n1 = false, achstemp = false;
     unsigned long long seed = 0;
     int nfreq = 3, repeat = 1, device = -1;
 };
@@ -83,6 +86,7 @@ bool parse(int argc, char **argv, Args &a)
         else if (s == "--mixed") a.mixed = true;
         else if (s == "--unmasked") a.unmasked = true;
         else if (s == "--acht-n1") a.acht_n1 = true;
+        else if (s == "--achstemp") a.achstemp = true;
         else if (s == "--repeat") { if (!(v = next())) return false; a.repeat = std::atoi(v); }
         else if (s == "--device") { if (!(v = next())) return false; a.device = std::atoi(v); }
         else return false;
@@ -199,21 +203,25 @@ int main(int argc, char **argv)
     p.to1 = to1; p.limittwo = limittwo; p.limitone = limitone; p.occ = 1.0;
 
     // outputs: host for FK_MEM_HOST, device (copied back) for FK_MEM_DEVICE
-    std::vector<fk_cdouble> achtemp(nfreq), asxtemp(nfreq), acht_n1(a.acht_n1 ? nb : 0);
+    std::vector<fk_cdouble> achtemp(nfreq), asxtemp(nfreq), acht_n1(a.acht_n1 ? nb : 0), achstemp(1);
     fk_cdouble *o_ach = achtemp.data(), *o_asx = asxtemp.data(), *o_n1 = a.acht_n1 ? acht_n1.data() : nullptr;
+    fk_cdouble *o_achs = a.achstemp ? achstemp.data() : nullptr;
     fk_cdouble *d_out = nullptr;
     if (a.random) {
-        hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * (2 * nfreq + nb)), "hipMalloc outputs");
+        hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * (2 * nfreq + nb + 1)), "hipMalloc outputs");
         o_ach = d_out;
         o_asx = d_out + nfreq;
         o_n1 = a.acht_n1 ? d_out + 2 * nfreq : nullptr;
+        o_achs = a.achstemp ? d_out + 2 * nfreq + nb : nullptr;
     }
+    fk_gpp_outputs outs;
+    outs.achtemp = o_ach; outs.asxtemp = o_asx; outs.acht_n1_loc = o_n1; outs.achstemp = o_achs;
     float kms = 0.f;
     double kms_sum = 0.0;
     opt.kernel_ms = &kms;
     const auto t0 = std::chrono::high_resolution_clock::now();
     for (int r = 0; r < a.repeat; ++r) {
-        int rc = gpp_kernel(&p, &opt, o_ach, o_asx, o_n1);
+        int rc = gpp_kernel_ex(&p, &opt, &outs);
         if (rc != FK_OK) fail("gpp_kernel", rc);
         kms_sum += kms;
     }
@@ -224,6 +232,8 @@ int main(int argc, char **argv)
         hip_check(hipMemcpy(asxtemp.data(), o_asx, sizeof(fk_cdouble) * nfreq, hipMemcpyDeviceToHost), "copy");
         if (a.acht_n1)
             hip_check(hipMemcpy(acht_n1.data(), o_n1, sizeof(fk_cdouble) * nb, hipMemcpyDeviceToHost), "copy");
+        if (a.achstemp)
+            hip_check(hipMemcpy(achstemp.data(), o_achs, sizeof(fk_cdouble), hipMemcpyDeviceToHost), "copy");
     }
 
     for (int iw = 0; iw < nfreq; ++iw)
@@ -236,6 +246,9 @@ int main(int argc, char **argv)
         for (int n1 = 0; n1 < nb; ++n1)
             std::cout << "acht_n1_loc[" << n1 << "] = " << std::setprecision(15)
                       << std::complex<double>(acht_n1[n1].re, acht_n1[n1].im) << std::endl;
+    if (a.achstemp)
+        std::cout << "achstemp = " << std::setprecision(15) << std::complex<double>(achstemp[0].re, achstemp[0].im)
+                  << std::endl;
     const double updates = (double)nb * ngpown * ncouls * nfreq;
     std::cout << std::setprecision(6) << "Kernel time = " << kms_sum / a.repeat << " ms per call ("
               << updates / (kms_sum / a.repeat * 1e-3) << " updates/s, " << a.repeat << " call"

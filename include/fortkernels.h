@@ -140,8 +140,29 @@ typedef struct {
 int gpp_kernel(const fk_gpp_problem *problem, const fk_options *options,
                fk_cdouble *achtemp, fk_cdouble *asxtemp, fk_cdouble *acht_n1_loc);
 
-/* Device scratch bytes gpp_kernel needs for this problem and options. */
+/* Device scratch bytes gpp_kernel (and gpp_kernel_ex) needs for this
+ * problem and options. */
 size_t gpp_kernel_workspace_size(const fk_gpp_problem *problem, const fk_options *options);
+
+/* gpp_kernel with its outputs in one struct and one more output, the
+ * static-CH remainder of the Fortran GPP program
+ * (gppkernel.deslippe.09.30.2014.f90:192-230), summed over every band:
+ *   achstemp = sum_n1 sum_my_igp 0.5 vcoul[igp] *
+ *              (- sum_ig aqsn[n1][ig] I_eps[my_igp][ig] conj(aqsm[n1][igp]))
+ * over the columns the map keeps.  It comes out of the GPP kernels' own
+ * tiles (B = the band-summed coefficient product, achstemp = -sum B I_eps),
+ * so it costs one complex product per element of the epilogue.  The C++
+ * program's reduce_achstemp (BGW-Kernels/gppKer.cpp:67-128) never writes its
+ * output, and its program drops the value; the Fortran's sum is the one
+ * computed.  achstemp may be NULL (then this is gpp_kernel). */
+typedef struct {
+    fk_cdouble *achtemp;       /* [nfreq] */
+    fk_cdouble *asxtemp;       /* [nfreq] */
+    fk_cdouble *acht_n1_loc;   /* [number_bands] or NULL */
+    fk_cdouble *achstemp;      /* one value or NULL */
+} fk_gpp_outputs;
+
+int gpp_kernel_ex(const fk_gpp_problem *problem, const fk_options *options, const fk_gpp_outputs *outputs);
 
 /* ------------------------------------------------------------------------
  * Batched GPP: many independent problems in one launch.  No counterpart in

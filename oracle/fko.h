@@ -20,6 +20,9 @@
  *                      ssxt_scht_solver on seeded inputs), brute-force
  *                      branch cases.
  *   fko_gpp_point    : golden vectors of the compiled ssxt_scht_solver.
+ *   fko_gpp_achstemp : the compiled achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:
+ *                      107-131), the same sum, run on the GPP inputs
+ *                      (golden vectors); closed form of constant inputs.
  *   fko_ff           : closed form (constant inputs), numpy/BLAS matmul on
  *                      small shapes, golden vectors of the compiled
  *                      achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:107-131).
@@ -82,6 +85,15 @@ int fko_gpp(int nb, int nv, int ncouls, int ngpown, int nw,
             int mask_on, int extended, int nthreads,
             int band_lo, int band_hi,
             fko_z *achtemp, fko_z *asxtemp, fko_z *acht_n1_loc);
+
+/* Static-CH remainder of the Fortran GPP program
+ * (gppkernel.deslippe.09.30.2014.f90:192-230), summed over every band:
+ *   achstemp = sum_n1 sum_my_igp 0.5 vcoul[igp] *
+ *              (- sum_ig aqsn[n1][ig] I_eps[my_igp][ig] conj(aqsm[n1][igp]))
+ * Long-double sums.  Returns 0 or -1. */
+int fko_gpp_achstemp(int nb, int ncouls, int ngpown, const fko_z *aqsm, const fko_z *aqsn,
+                     const fko_z *I_eps, const double *vcoul, const int32_t *inv_igp_index,
+                     const int32_t *indinv, int nthreads, fko_z *achstemp);
 
 /* Resolve the column map exactly as BGW-Kernels/gppKer.cpp:322-328:
  * igp_out[my_igp] = igp, or -1 when the reference skips the column.

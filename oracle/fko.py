@@ -54,6 +54,8 @@ def lib() -> ctypes.CDLL:
         L.fko_resolve_igp.argtypes = [I, I, P, P, P]
         L.fko_ff.restype = I
         L.fko_ff.argtypes = [I, I, I, I] + [P] * 6 + [I, I] + [P] * 3
+        L.fko_gpp_achstemp.restype = I
+        L.fko_gpp_achstemp.argtypes = [I, I, I] + [P] * 6 + [I, P]
         L.fko_ff_full.restype = I
         L.fko_ff_full.argtypes = [ctypes.POINTER(_FfFullProblem), I, I, ctypes.POINTER(_FfFullOut)]
         _lib = L
@@ -206,6 +208,18 @@ def gpp_point(wxt, wtilde, I_eps, to1=TO1, limittwo=LIMITTWO, limitone=LIMITONE)
     br = lib().fko_gpp_point(wxt, _Z(wtilde.real, wtilde.imag), _Z(I_eps.real, I_eps.imag),
                              to1, limittwo, limitone, _p(ssx), _p(sch))
     return br, complex(ssx[0]), complex(sch[0])
+
+
+def gpp_achstemp(p, nthreads=0):
+    """Static-CH remainder of the Fortran GPP program over every band
+    (gppkernel.deslippe.09.30.2014.f90:192-230) for a GppProblem."""
+    out = np.zeros(1, np.complex128)
+    arrs = [_c(p.aqsm, np.complex128), _c(p.aqsn, np.complex128), _c(p.I_eps, np.complex128),
+            _c(p.vcoul, np.float64), _c(p.inv_igp_index, np.int32), _c(p.indinv, np.int32)]
+    rc = lib().fko_gpp_achstemp(p.nb, p.ncouls, p.ngpown, *[_p(a) for a in arrs], int(nthreads), _p(out))
+    if rc != 0:
+        raise ValueError("oracle rejected the problem")
+    return complex(out[0])
 
 
 def resolve_igp(ncouls, ngpown, inv, indinv):

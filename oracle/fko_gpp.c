@@ -291,3 +291,53 @@ int fko_gpp(int nb, int nv, int ncouls, int ngpown, int nw,
     free(col_ach); free(col_asx); free(col_n1); free(igp_of);
     return 0;
 }
+
+/* Static-CH remainder of the Fortran GPP program,
+ * gppkernel.deslippe.09.30.2014.f90:192-230:
+ *   schstemp = sum_ig -aqsntemp(ig,n1) * I_eps_array(ig,my_igp) * CONJG(aqsmtemp(igp,n1))
+ *   achstemp += schstemp * vcoul(igp) * 0.5            for every band n1
+ * over the columns the map keeps (:201-204; the C++ map of
+ * BGW-Kernels/gppKer.cpp:322-328, as everywhere here).  Terms in FP64 in the
+ * Fortran's order, sums in long double.  (The C++ reduce_achstemp,
+ * BGW-Kernels/gppKer.cpp:67-128, never writes its output and indexes a
+ * zero-length array; it has no observable result to restate.) */
+int fko_gpp_achstemp(int nb, int ncouls, int ngpown, const z *aqsm, const z *aqsn, const z *I_eps,
+                     const double *vcoul, const int32_t *inv_igp_index, const int32_t *indinv, int nthreads,
+                     z *achstemp)
+{
+    if (nb < 0 || ncouls <= 0 || ngpown <= 0 || !achstemp) return -1;
+    int32_t *igp_of = (int32_t *)malloc(sizeof(int32_t) * (size_t)ngpown);
+    if (!igp_of) return -1;
+    if (fko_resolve_igp(ncouls, ngpown, inv_igp_index, indinv, igp_of)) { free(igp_of); return -1; }
+    zl *col = (zl *)calloc((size_t)ngpown, sizeof(zl));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+#endif
+    for (int my_igp = 0; my_igp < ngpown; ++my_igp) {
+        const int igp = igp_of[my_igp];
+        if (igp < 0) continue;
+        const z *ie = I_eps + (size_t)my_igp * ncouls;
+        zl acc = {0, 0};
+        for (int n1 = 0; n1 < nb; ++n1) {
+            const z g1 = zconj(aqsm[(size_t)n1 * ncouls + igp]);
+            const z *an = aqsn + (size_t)n1 * ncouls;
+            zl s = {0, 0};
+            for (int ig = 0; ig < ncouls; ++ig) {
+                const z t = zmul(zmul(an[ig], ie[ig]), g1);
+                s.re -= t.re;
+                s.im -= t.im;
+            }
+            const double v = vcoul[igp] * 0.5;
+            acc.re += s.re * v;
+            acc.im += s.im * v;
+        }
+        col[my_igp] = acc;
+    }
+    long double r = 0, i = 0;
+    for (int my_igp = 0; my_igp < ngpown; ++my_igp) { r += col[my_igp].re; i += col[my_igp].im; }
+    *achstemp = zmk((double)r, (double)i);
+    free(col);
+    free(igp_of);
+    return 0;
+}

@@ -17,6 +17,11 @@ Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
                    (FFKer_cori/ffKernel.cpp:107-131, frequency index 1)
   ff_main.npz      achsDtemp, asxDtemp[0] and achDtemp_cor[0] printed by
                    the reference's ffKernel program (constant inputs)
+  gpp_achstemp.npz GPP inputs + the static-CH remainder achstemp of the
+                   Fortran GPP program (gppkernel.deslippe.09.30.2014.f90:
+                   192-230), written by the reference's achsDtemp_Kernel
+                   (FFKer_cori/ffKernel.cpp:107-131), which computes the same
+                   sum, run with the GPP I_eps at its frequency index 1
   ff_full_<case>.npz  inputs + achsDtemp, asxDtemp, achDtemp, ach2Dtemp,
                    achDtemp_cor, achDtemp_corb written by
                    oracle/_ref/ref_ff_harness full, i.e. by the reference's
@@ -241,6 +246,24 @@ def ff_full_cases():
         print(f"ff_full_{name}:", {k: np.round(v, 4) for k, v in out.items()})
 
 
+def gpp_achstemp_case():
+    """achsDtemp_Kernel sums -0.5 vcoul[igp] aqsn[n1][ig] conj(aqsm[n1][igp])
+    I_epsR[1][my_igp][ig] over n1, my_igp, ig: the Fortran GPP achstemp when
+    I_epsR[1] is the GPP I_eps.  Its column map has no skip rule, so the map
+    here is a permutation."""
+    p = fko.gpp_random(7, 2, 48, 6, seed=17)
+    inv = np.array([5, 0, 3, 1, 4, 2], np.int32) * 8 + np.array([1, 7, 2, 0, 5, 3], np.int32)
+    p.inv_igp_index = inv
+    other = fko.fill_complex(p.ngpown * p.ncouls, 99, 3, (-0.5, 0.5), (-0.5, 0.5)).reshape(p.ngpown, p.ncouls)
+    q = fko.FfProblem(p.nb, p.ncouls, p.ngpown, 2, p.aqsm, p.aqsn, np.stack([other, p.I_eps]), p.vcoul,
+                      p.inv_igp_index, p.indinv)
+    achs = run_ff_harness(q)
+    np.savez_compressed(os.path.join(HERE, "gpp_achstemp.npz"), nb=p.nb, nv=p.nv, ncouls=p.ncouls,
+                        ngpown=p.ngpown, wx=p.wx, aqsm=p.aqsm, aqsn=p.aqsn, I_eps=p.I_eps, wtilde=p.wtilde,
+                        vcoul=p.vcoul, inv_igp_index=p.inv_igp_index, indinv=p.indinv, achstemp=achs)
+    print("gpp_achstemp:", achs)
+
+
 def main():
     for exe in ("ref_gpp_harness", "ref_ff_harness", "gppKer", "ffKernel"):
         if not os.path.exists(os.path.join(REF_BIN, exe)):
@@ -253,6 +276,7 @@ def main():
     reference_main()
     ff_cases()
     ff_full_cases()
+    gpp_achstemp_case()
 
 
 if __name__ == "__main__":

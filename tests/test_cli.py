@@ -71,6 +71,25 @@ def test_gppker_random_matches_library(fk, fko):
 
 
 @pytest.mark.gpu
+def test_gppker_achstemp(fko):
+    """--achstemp prints the Fortran static-CH remainder; with the constant
+    inputs it is -0.5 nb ngpown ncouls (0.25+0.25i)."""
+    nb, nv, C, npg = 16, 2, 1024, 20
+    r = _run("gppKer", nb, nv, C, npg, "--achstemp")
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"^achstemp = \(([^,]+),([^)]+)\)$", r.stdout, re.M)
+    got = complex(float(m.group(1)), float(m.group(2)))
+    want = -0.5 * nb * (C // npg) * C * (0.25 + 0.25j)
+    assert abs(got - want) <= 1e-13 * abs(want)
+    r = _run("gppKer", nb, nv, C, npg, "--random", 2, "--achstemp")
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"^achstemp = \(([^,]+),([^)]+)\)$", r.stdout, re.M)
+    p = fko.gpp_random(nb, nv, C, C // npg, seed=2)
+    want = fko.gpp_achstemp(p)
+    assert abs(complex(float(m.group(1)), float(m.group(2))) - want) <= 1e-12 * abs(want)
+
+
+@pytest.mark.gpu
 def test_ffker_matches_reference_program(fko):
     """achsDtemp as the C++ program prints it (frequency index 1); asxDtemp,
     achDtemp_cor and the rest follow the Fortran program, checked against the

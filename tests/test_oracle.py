@@ -207,3 +207,37 @@ def test_rng_known_answers_and_range(fko):
     z = fko.fill_complex(8, 9, 3, (-0.5, 0.5), (0.0, 1.0))
     assert z[0].real == fko.lib().fko_uniform(9, 3, 0, -0.5, 0.5)
     assert z[0].imag == fko.lib().fko_uniform(9, 3, 1, 0.0, 1.0)
+
+
+def test_gpp_achstemp_matches_reference_kernel(fko):
+    """The Fortran GPP static-CH remainder (gppkernel.deslippe.09.30.2014.f90:
+    192-230) against the compiled achsDtemp_Kernel, the same sum, run on the
+    GPP inputs (tests/golden/make_golden.py)."""
+    d = np.load(os.path.join(GOLDEN, "gpp_achstemp.npz"))
+    p = fko.GppProblem(int(d["nb"]), int(d["nv"]), int(d["ncouls"]), int(d["ngpown"]), d["wx"], d["aqsm"],
+                       d["aqsn"], d["I_eps"], d["wtilde"], d["vcoul"], d["inv_igp_index"], d["indinv"])
+    ref = complex(d["achstemp"])
+    assert abs(fko.gpp_achstemp(p) - ref) <= 1e-13 * abs(ref)
+    # each factor enters once: conjugating aqsm alone changes the value
+    q = fko.GppProblem(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, np.conj(p.aqsm), p.aqsn, p.I_eps, p.wtilde,
+                       p.vcoul, p.inv_igp_index, p.indinv)
+    assert abs(fko.gpp_achstemp(q) - ref) > 1e-3 * abs(ref)
+
+
+@pytest.mark.parametrize("shape", [(8, 2, 64, 4), (5, 1, 40, 8)])
+def test_gpp_achstemp_closed_form(fko, shape):
+    """Constant inputs (0.5+0.5i, vcoul = 1): every term is
+    -(0.5+0.5i)(0.5+0.5i)(0.5-0.5i) = -(0.25+0.25i), so
+    achstemp = -0.5 nb ngpown ncouls (0.25+0.25i); a skipped column drops
+    its nb ncouls terms."""
+    nb, nv, C, npg = shape
+    p = fko.gpp_constant(nb, nv, C, npg)
+    G = p.ngpown
+    want = -0.5 * nb * G * C * (0.25 + 0.25j)
+    assert abs(fko.gpp_achstemp(p) - want) <= 1e-15 * abs(want)
+    ind = p.indinv.copy()
+    igp = fko.resolve_igp(C, G, p.inv_igp_index, ind)
+    ind[p.inv_igp_index[1]] = -1
+    p.indinv = ind
+    assert abs(fko.gpp_achstemp(p) - want * (G - 1) / G) <= 1e-15 * abs(want)
+    assert igp[1] >= 0
