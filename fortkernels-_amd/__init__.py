@@ -34,7 +34,9 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe",
            "gpp_batch_create", "gpp_batch_run", "gpp_batch_config", "gpp_batch_workspace_bytes",
            "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size",
-           "gpp_kernel_ex"]
+           "gpp_kernel_ex", "fk_gpp_instance_write", "fk_ff_instance_write", "fk_instance_open",
+           "fk_instance_kind", "fk_instance_complex_bytes", "fk_instance_gpp", "fk_instance_ff", "fk_instance_close",
+           "fk_instance_list_read", "fk_instance_list_free"]
 
 
 class FkError(RuntimeError):
@@ -113,6 +115,27 @@ def _load():
     L.ff_kernel.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options), P, P, P]
     L.ff_kernel_workspace_size.restype = SZ
     L.ff_kernel_workspace_size.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options)]
+    L.fk_gpp_instance_write.restype = ctypes.c_int
+    L.fk_gpp_instance_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(GppProblem), I32]
+    L.fk_ff_instance_write.restype = ctypes.c_int
+    L.fk_ff_instance_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(FfFullProblem)]
+    L.fk_instance_open.restype = ctypes.c_int
+    L.fk_instance_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
+    L.fk_instance_kind.restype = ctypes.c_int
+    L.fk_instance_kind.argtypes = [P]
+    L.fk_instance_complex_bytes.restype = I32
+    L.fk_instance_complex_bytes.argtypes = [P]
+    L.fk_instance_gpp.restype = ctypes.POINTER(GppProblem)
+    L.fk_instance_gpp.argtypes = [P]
+    L.fk_instance_ff.restype = ctypes.POINTER(FfFullProblem)
+    L.fk_instance_ff.argtypes = [P]
+    L.fk_instance_close.restype = None
+    L.fk_instance_close.argtypes = [P]
+    L.fk_instance_list_read.restype = ctypes.c_int
+    L.fk_instance_list_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_char_p)),
+                                        ctypes.POINTER(I32)]
+    L.fk_instance_list_free.restype = None
+    L.fk_instance_list_free.argtypes = [ctypes.POINTER(ctypes.c_char_p), I32]
     L.gpp_kernel_ex.restype = ctypes.c_int
     L.gpp_kernel_ex.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), ctypes.POINTER(GppOutputs)]
     L.ff_kernel_full.restype = ctypes.c_int
@@ -457,6 +480,113 @@ def ff_full_problem(p, **kw):
                           p.I_epsA, p.vcoul, p.inv_igp_index, p.indinv, p.ekq, p.dFreqGrid, p.pref, p.dFreqBrd,
                           pref_zb=p.pref_zb, freqevalmin=p.freqevalmin, freqevalstep=p.freqevalstep, occ=p.occ,
                           achs_freq=p.achs_freq, **kw)
+
+
+# --------------------------------------------------------------------------
+# instance files (fk_io.cpp)
+# --------------------------------------------------------------------------
+class _Fields:
+    """Plain attribute bag for the arrays and sizes of an instance."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def gpp_instance_write(path, p, complex64=False):
+    """Write a GPP problem (an object with the oracle GppProblem's fields,
+    numpy arrays) to an instance file."""
+    cdt = np.complex64 if complex64 else np.complex128
+    keep = [np.ascontiguousarray(p.wx, np.float64), np.ascontiguousarray(p.inv_igp_index, np.int32),
+            np.ascontiguousarray(p.indinv, np.int32), np.ascontiguousarray(p.vcoul, np.float64)]
+    cx = [np.ascontiguousarray(getattr(p, k), cdt) for k in ("aqsm", "aqsn", "I_eps", "wtilde")]
+    prob = GppProblem(int(p.nb), int(p.nv), int(p.ncouls), int(p.ngpown), len(keep[0]), keep[0].ctypes.data,
+                      cx[0].ctypes.data, cx[1].ctypes.data, cx[2].ctypes.data, cx[3].ctypes.data,
+                      keep[3].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, float(p.to1),
+                      float(p.limittwo), float(p.limitone), float(p.occ))
+    _check(lib.fk_gpp_instance_write(os.fsencode(path), ctypes.byref(prob), 1 if complex64 else 0))
+
+
+def ff_instance_write(path, p):
+    """Write an FF evaluation problem (the oracle FfFullProblem's fields)."""
+    host = [np.ascontiguousarray(p.inv_igp_index, np.int32), np.ascontiguousarray(p.indinv, np.int32),
+            np.ascontiguousarray(p.ekq, np.float64), np.ascontiguousarray(p.dFreqGrid, np.float64),
+            np.ascontiguousarray(p.pref, np.float64), np.ascontiguousarray(p.dFreqBrd, np.complex128)]
+    cx = [np.ascontiguousarray(getattr(p, k), np.complex128) for k in ("aqsm", "aqsn", "I_epsR", "I_epsA")]
+    v = np.ascontiguousarray(p.vcoul, np.float64)
+    prob = _ff_full_struct(p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval, *cx, v, host,
+                           (p.pref_zb, p.freqevalmin, p.freqevalstep, p.occ, p.achs_freq))
+    _check(lib.fk_ff_instance_write(os.fsencode(path), ctypes.byref(prob)))
+
+
+def _view(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    buf = (ctypes.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
+class Instance:
+    """An instance file opened with fk_instance_open: .kind ("gpp" / "ff"),
+    .problem (the arrays as read-only numpy views of the mapping, with the
+    oracle problem classes' field names) and .raw (the C problem struct).
+    Close it (or use it as a context manager) only after the last use of
+    the views."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        _check(lib.fk_instance_open(os.fsencode(path), ctypes.byref(h)))
+        self._h = h
+        kind = lib.fk_instance_kind(h)
+        if kind == 1:
+            self.kind = "gpp"
+            q = lib.fk_instance_gpp(h).contents
+            cdt = np.complex64 if lib.fk_instance_complex_bytes(h) == 8 else np.complex128
+            nb, C, G = q.number_bands, q.ncouls, q.ngpown
+            self.problem = _Fields(
+                nb=nb, nv=q.nvband, ncouls=C, ngpown=G, wx=_view(q.wx, q.nfreq, np.float64),
+                aqsm=_view(q.aqsmtemp, nb * C, cdt).reshape(nb, C), aqsn=_view(q.aqsntemp, nb * C, cdt).reshape(nb, C),
+                I_eps=_view(q.I_eps_array, G * C, cdt).reshape(G, C),
+                wtilde=_view(q.wtilde_array, G * C, cdt).reshape(G, C), vcoul=_view(q.vcoul, C, np.float64),
+                inv_igp_index=_view(q.inv_igp_index, G, np.int32), indinv=_view(q.indinv, C + 1, np.int32),
+                to1=q.to1, limittwo=q.limittwo, limitone=q.limitone, occ=q.occ)
+            self.raw = q
+        else:
+            self.kind = "ff"
+            q = lib.fk_instance_ff(h).contents
+            nb, C, G, nf = q.number_bands, q.ncouls, q.ngpown, q.nfreq
+            self.problem = _Fields(
+                nb=nb, nv=q.nvband, ncouls=C, ngpown=G, nfreq=nf, nfreqeval=q.nfreqeval,
+                aqsm=_view(q.aqsmtemp, nb * C, np.complex128).reshape(nb, C),
+                aqsn=_view(q.aqsntemp, nb * C, np.complex128).reshape(nb, C),
+                I_epsR=_view(q.I_epsR_array, nf * G * C, np.complex128).reshape(nf, G, C),
+                I_epsA=_view(q.I_epsA_array, nf * G * C, np.complex128).reshape(nf, G, C),
+                vcoul=_view(q.vcoul, C, np.float64), inv_igp_index=_view(q.inv_igp_index, G, np.int32),
+                indinv=_view(q.indinv, C + 1, np.int32), ekq=_view(q.ekq, nb, np.float64),
+                dFreqGrid=_view(q.dFreqGrid, nf, np.float64), pref=_view(q.pref, nf, np.float64),
+                dFreqBrd=_view(q.dFreqBrd, nf, np.complex128), pref_zb=q.pref_zb, freqevalmin=q.freqevalmin,
+                freqevalstep=q.freqevalstep, occ=q.occ, achs_freq=q.achs_freq)
+            self.raw = q
+
+    def close(self):
+        if self._h is not None:
+            lib.fk_instance_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def instance_list_read(path):
+    arr = ctypes.POINTER(ctypes.c_char_p)()
+    n = ctypes.c_int32(0)
+    _check(lib.fk_instance_list_read(os.fsencode(path), ctypes.byref(arr), ctypes.byref(n)))
+    try:
+        return [os.fsdecode(arr[i]) for i in range(n.value)]
+    finally:
+        lib.fk_instance_list_free(arr, n)
 
 
 # --------------------------------------------------------------------------

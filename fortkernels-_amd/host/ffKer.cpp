@@ -32,6 +32,10 @@
 //                    evaluation existed; with --both also for I_epsA
 //   --repeat K       run K times, print the mean device time
 //   --device D       HIP device ordinal
+//   --dump FILE      write the inputs of this run (the evaluation's) to an
+//                    instance file
+//   --input FILE     run the FF problem of an instance file (its own axes,
+//                    grid, weights and maps) instead of the six numbers
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -49,7 +53,8 @@ namespace {
 void usage_params()
 {
     std::cout << "./a.out number_bands nvband ncouls ngpown nFreq nfreqeval "
-                 "[--random SEED] [--achs-freq F] [--mixed] [--contraction [--both]] [--repeat K] [--device D]"
+                 "[--random SEED] [--achs-freq F] [--mixed] [--contraction [--both]] [--repeat K] [--device D] "
+                 "[--dump FILE]\n./a.out --input FILE [--mixed] [--repeat K] [--device D]"
               << std::endl;
 }
 
@@ -70,6 +75,7 @@ void hip_check(hipError_t e, const char *what)
 struct Args {
     int nb = 0, nv = 0, ncouls = 0, ngpown = 0, nfreq = 0, nfreqeval = 0;
     bool random = false, both = false, mixed = false, contraction = false;
+    std::string input, dump;
     unsigned long long seed = 0;
     int achs_freq = 1, repeat = 1, device = -1;
 };
@@ -89,8 +95,11 @@ bool parse(int argc, char **argv, Args &a)
         else if (s == "--contraction") a.contraction = true;
         else if (s == "--repeat") { if (!(v = next())) return false; a.repeat = std::atoi(v); }
         else if (s == "--device") { if (!(v = next())) return false; a.device = std::atoi(v); }
+        else if (s == "--input") { if (!(v = next())) return false; a.input = v; }
+        else if (s == "--dump") { if (!(v = next())) return false; a.dump = v; }
         else return false;
     }
+    if (!a.input.empty()) return pos.empty() && a.repeat >= 1 && !a.random && !a.contraction && a.dump.empty();
     if (pos.size() != 6) return false;
     int *dst[] = {&a.nb, &a.nv, &a.ncouls, &a.ngpown, &a.nfreq, &a.nfreqeval};
     for (int k = 0; k < 6; ++k) *dst[k] = std::atoi(pos[k].c_str());
@@ -99,12 +108,112 @@ bool parse(int argc, char **argv, Args &a)
 
 void print_c(const char *name, fk_cdouble z) { std::printf("%s = ( %f, %f) \n", name, z.re, z.im); }
 
+// One ff_kernel_full call (or K), then the program's output lines.
+void run_full_and_print(const fk_ff_full_problem &p, fk_options opt, const Args &a)
+{
+    const int nb = p.number_bands, C = p.ncouls, G = p.ngpown, nf = p.nfreq, ne = p.nfreqeval;
+    const bool on_dev = opt.memory == FK_MEM_DEVICE;
+    float kms = 0.f;
+    opt.kernel_ms = &kms;
+    // outputs: achs, then five [ne] arrays
+    const size_t nout = 1 + 5 * (size_t)ne;
+    std::vector<fk_cdouble> h(nout);
+    fk_cdouble *d_out = nullptr;
+    fk_cdouble *o = h.data();
+    if (on_dev) {
+        hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * nout), "hipMalloc outputs");
+        o = d_out;
+    }
+    fk_ff_full_outputs out;
+    out.achsDtemp = o; out.asxDtemp = o + 1; out.achDtemp = o + 1 + ne; out.achDtemp_cor = o + 1 + 2 * ne;
+    out.achDtemp_corb = o + 1 + 3 * ne; out.ach2Dtemp = o + 1 + 4 * ne; out.schDt = nullptr;
+    double ms_sum = 0.0;
+    const auto t_k = std::chrono::high_resolution_clock::now();
+    for (int r = 0; r < a.repeat; ++r) {
+        const int rc = ff_kernel_full(&p, &opt, &out);
+        if (rc != FK_OK) fail("ff_kernel_full", rc);
+        ms_sum += kms;
+    }
+    const std::chrono::duration<double> kernel_time = std::chrono::high_resolution_clock::now() - t_k;
+    if (on_dev) {
+        hip_check(hipMemcpy(h.data(), d_out, sizeof(fk_cdouble) * nout, hipMemcpyDeviceToHost), "copy");
+        (void)hipFree(d_out);
+    }
+    const fk_cdouble *asx = &h[1], *ach = asx + ne, *cor = ach + ne, *corb = cor + ne, *a2 = corb + ne;
+    print_c("achsDtemp", h[0]);
+    print_c("asxDtemp", asx[0]);
+    print_c("achDtemp_cor", cor[0]);
+    std::printf(" Answer: (%.15e,%.15e) (%.15e,%.15e) (%.15e,%.15e)\n", cor[0].re, cor[0].im, ach[0].re,
+                ach[0].im, asx[0].re, asx[0].im);
+    std::printf("achsDtemp (full precision) = (%.15e, %.15e)\n", h[0].re, h[0].im);
+    const char *names[5] = {"asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp"};
+    const fk_cdouble *vals[5] = {asx, ach, cor, corb, a2};
+    for (int k = 0; k < 5; ++k)
+        for (int iw = 0; iw < ne; ++iw)
+            std::printf("%s[%d] = (%.15e,%.15e)\n", names[k], iw, vals[k][iw].re, vals[k][iw].im);
+    const double flops = 8.0 * (double)G * nf * nb * C;
+    const double ms = ms_sum / a.repeat;
+    std::printf("Device time = %.4f ms per call (both contractions and the evaluation; the I_epsR "
+                "contraction alone is %.3e flops, %.2f TFLOP/s over the whole call; %d call%s)\n",
+                ms, flops, flops / (ms * 1e-3) / 1e12, a.repeat, a.repeat > 1 ? "s" : "");
+    std::cout << "********** Total Kernel Time **********= " << kernel_time.count() << " secs" << std::endl;
+}
+
+// The inputs of this run as an instance file (device arrays copied back).
+void dump_instance(const std::string &path, const fk_ff_full_problem &p, bool on_dev)
+{
+    fk_ff_full_problem q = p;
+    const size_t nbc = (size_t)p.number_bands * p.ncouls, xc = (size_t)p.nfreq * p.ngpown * p.ncouls;
+    std::vector<fk_cdouble> m, n, r, x;
+    std::vector<double> v;
+    if (on_dev) {
+        m.resize(nbc); n.resize(nbc); r.resize(xc); x.resize(xc); v.resize(p.ncouls);
+        hip_check(hipMemcpy(m.data(), p.aqsmtemp, nbc * sizeof(fk_cdouble), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(n.data(), p.aqsntemp, nbc * sizeof(fk_cdouble), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(r.data(), p.I_epsR_array, xc * sizeof(fk_cdouble), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(x.data(), p.I_epsA_array, xc * sizeof(fk_cdouble), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(v.data(), p.vcoul, sizeof(double) * v.size(), hipMemcpyDeviceToHost), "dump copy");
+        q.aqsmtemp = m.data(); q.aqsntemp = n.data(); q.I_epsR_array = r.data(); q.I_epsA_array = x.data();
+        q.vcoul = v.data();
+    }
+    const int rc = fk_ff_instance_write(path.c_str(), &q);
+    if (rc != FK_OK) fail("fk_ff_instance_write", rc);
+    std::cout << "Inputs written to " << path << std::endl;
+}
+
+// --input: the problem of an instance file, host memory.
+int run_input(const Args &a)
+{
+    fk_instance *inst = nullptr;
+    int rc = fk_instance_open(a.input.c_str(), &inst);
+    if (rc != FK_OK) fail("fk_instance_open", rc);
+    const fk_ff_full_problem *p = fk_instance_ff(inst);
+    if (!p) {
+        std::cerr << "ffKer: " << a.input << " holds a GPP problem, not an FF one" << std::endl;
+        fk_instance_close(inst);
+        return 2;
+    }
+    if (a.device >= 0) hip_check(hipSetDevice(a.device), "hipSetDevice");
+    std::cout << "Instance = " << a.input << "\n number_bands = " << p->number_bands << "\n nvband = " << p->nvband
+              << "\n ncouls = " << p->ncouls << "\n ngpown = " << p->ngpown << "\n nFreq = " << p->nfreq
+              << "\n nfreqeval = " << p->nfreqeval << std::endl;
+    fk_options opt = fk_default_options();
+    opt.memory = FK_MEM_HOST;
+    opt.precision = a.mixed ? FK_PREC_MIXED : FK_PREC_FP64;
+    opt.device = a.device;
+    run_full_and_print(*p, opt, a);
+    fk_instance_close(inst);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     Args a;
-    if (!parse(argc, argv, a)) {
+    const bool parsed = parse(argc, argv, a);
+    if (parsed && !a.input.empty()) return run_input(a);
+    if (!parsed) {
         std::cout << "Incorrect Parameters!!! The correct form is " << std::endl;
         usage_params();
         return 0;
@@ -201,48 +310,8 @@ int main(int argc, char **argv)
         p.ekq = ekq.data(); p.dFreqGrid = grid.data(); p.pref = pref.data(); p.dFreqBrd = brd.data();
         p.pref_zb = 0.5 / 3.14; p.freqevalmin = 0.0; p.freqevalstep = 0.5; p.occ = 1.0;
         p.achs_freq = a.achs_freq;
-        // outputs: achs, then five [ne] arrays
-        const size_t nout = 1 + 5 * (size_t)ne;
-        std::vector<fk_cdouble> h(nout);
-        fk_cdouble *d_out = nullptr;
-        fk_cdouble *o = h.data();
-        if (a.random) {
-            hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * nout), "hipMalloc outputs");
-            o = d_out;
-        }
-        fk_ff_full_outputs out;
-        out.achsDtemp = o; out.asxDtemp = o + 1; out.achDtemp = o + 1 + ne; out.achDtemp_cor = o + 1 + 2 * ne;
-        out.achDtemp_corb = o + 1 + 3 * ne; out.ach2Dtemp = o + 1 + 4 * ne; out.schDt = nullptr;
-        double ms_sum = 0.0;
-        const auto t_k = std::chrono::high_resolution_clock::now();
-        for (int r = 0; r < a.repeat; ++r) {
-            const int rc = ff_kernel_full(&p, &opt, &out);
-            if (rc != FK_OK) fail("ff_kernel_full", rc);
-            ms_sum += kms;
-        }
-        const std::chrono::duration<double> kernel_time = std::chrono::high_resolution_clock::now() - t_k;
-        if (a.random) {
-            hip_check(hipMemcpy(h.data(), d_out, sizeof(fk_cdouble) * nout, hipMemcpyDeviceToHost), "copy");
-            (void)hipFree(d_out);
-        }
-        const fk_cdouble *asx = &h[1], *ach = asx + ne, *cor = ach + ne, *corb = cor + ne, *a2 = corb + ne;
-        print_c("achsDtemp", h[0]);
-        print_c("asxDtemp", asx[0]);
-        print_c("achDtemp_cor", cor[0]);
-        std::printf(" Answer: (%.15e,%.15e) (%.15e,%.15e) (%.15e,%.15e)\n", cor[0].re, cor[0].im, ach[0].re,
-                    ach[0].im, asx[0].re, asx[0].im);
-        std::printf("achsDtemp (full precision) = (%.15e, %.15e)\n", h[0].re, h[0].im);
-        const char *names[5] = {"asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp"};
-        const fk_cdouble *vals[5] = {asx, ach, cor, corb, a2};
-        for (int k = 0; k < 5; ++k)
-            for (int iw = 0; iw < ne; ++iw)
-                std::printf("%s[%d] = (%.15e,%.15e)\n", names[k], iw, vals[k][iw].re, vals[k][iw].im);
-        const double flops = 8.0 * (double)G * nf * nb * C;
-        const double ms = ms_sum / a.repeat;
-        std::printf("Device time = %.4f ms per call (both contractions and the evaluation; the I_epsR "
-                    "contraction alone is %.3e flops, %.2f TFLOP/s over the whole call; %d call%s)\n",
-                    ms, flops, flops / (ms * 1e-3) / 1e12, a.repeat, a.repeat > 1 ? "s" : "");
-        std::cout << "********** Total Kernel Time **********= " << kernel_time.count() << " secs" << std::endl;
+        if (!a.dump.empty()) dump_instance(a.dump, p, a.random);
+        run_full_and_print(p, opt, a);
     } else {
         const size_t nout = (size_t)nb * nf;
         std::vector<fk_cdouble> schDt[2], achs(2);

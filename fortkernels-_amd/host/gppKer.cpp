@@ -25,6 +25,10 @@
 //                   192-230; gpp_kernel_ex)
 //   --repeat K      run the kernel K times and print the mean kernel time
 //   --device D      HIP device ordinal
+//   --dump FILE     write the inputs of this run to an instance file
+//   --input FILE    run the problem of an instance file (fk_instance_open)
+//                   instead of the four numbers: sizes, wx, column maps and
+//                   arrays come from the file; complex64 files run mixed
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -47,6 +51,7 @@ void usage()
     std::cout << "The correct form of input is : " << std::endl;
     std::cout << " ./a.out <number_bands> <number_valence_bands> <number_plane_waves> <nodes_per_mpi_group> "
                  "[--random SEED] [--near-pole] [--nfreq N] [--mixed] [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
+                 "[--device D] [--dump FILE]\n ./a.out --input FILE [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
                  "[--device D]"
               << std::endl;
 }
@@ -70,6 +75,7 @@ struct Args {
     bool random = false, near_pole = false, mixed = false, unmasked = false, acht_n1 = false, achstemp = false;
     unsigned long long seed = 0;
     int nfreq = 3, repeat = 1, device = -1;
+    std::string input, dump;
 };
 
 bool parse(int argc, char **argv, Args &a)
@@ -89,14 +95,130 @@ bool parse(int argc, char **argv, Args &a)
         else if (s == "--achstemp") a.achstemp = true;
         else if (s == "--repeat") { if (!(v = next())) return false; a.repeat = std::atoi(v); }
         else if (s == "--device") { if (!(v = next())) return false; a.device = std::atoi(v); }
+        else if (s == "--input") { if (!(v = next())) return false; a.input = v; }
+        else if (s == "--dump") { if (!(v = next())) return false; a.dump = v; }
         else return false;
     }
+    if (!a.input.empty()) return pos.empty() && a.repeat >= 1 && !a.random && a.dump.empty();
     if (pos.size() != 4) return false;
     a.nb = std::atoi(pos[0].c_str());
     a.nv = std::atoi(pos[1].c_str());
     a.ncouls = std::atoi(pos[2].c_str());
     a.npg = std::atoi(pos[3].c_str());
     return a.repeat >= 1;
+}
+
+// One gpp_kernel_ex call (or K), then the reference's output lines.
+void run_and_print(const fk_gpp_problem &p, fk_options opt, const Args &a)
+{
+    const int nb = p.number_bands, nfreq = p.nfreq, ngpown = p.ngpown, ncouls = p.ncouls;
+    const bool on_dev = opt.memory == FK_MEM_DEVICE;
+    // outputs: host for FK_MEM_HOST, device (copied back) for FK_MEM_DEVICE
+    std::vector<fk_cdouble> achtemp(nfreq), asxtemp(nfreq), acht_n1(a.acht_n1 ? nb : 0), achstemp(1);
+    fk_cdouble *o_ach = achtemp.data(), *o_asx = asxtemp.data(), *o_n1 = a.acht_n1 ? acht_n1.data() : nullptr;
+    fk_cdouble *o_achs = a.achstemp ? achstemp.data() : nullptr;
+    fk_cdouble *d_out = nullptr;
+    if (on_dev) {
+        hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * (2 * nfreq + nb + 1)), "hipMalloc outputs");
+        o_ach = d_out;
+        o_asx = d_out + nfreq;
+        o_n1 = a.acht_n1 ? d_out + 2 * nfreq : nullptr;
+        o_achs = a.achstemp ? d_out + 2 * nfreq + nb : nullptr;
+    }
+    fk_gpp_outputs outs;
+    outs.achtemp = o_ach; outs.asxtemp = o_asx; outs.acht_n1_loc = o_n1; outs.achstemp = o_achs;
+    float kms = 0.f;
+    double kms_sum = 0.0;
+    opt.kernel_ms = &kms;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    for (int r = 0; r < a.repeat; ++r) {
+        int rc = gpp_kernel_ex(&p, &opt, &outs);
+        if (rc != FK_OK) fail("gpp_kernel", rc);
+        kms_sum += kms;
+    }
+    if (on_dev) hip_check(hipDeviceSynchronize(), "gpp_kernel");
+    const std::chrono::duration<double> elapsed = std::chrono::high_resolution_clock::now() - t0;
+    if (on_dev) {
+        hip_check(hipMemcpy(achtemp.data(), o_ach, sizeof(fk_cdouble) * nfreq, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpy(asxtemp.data(), o_asx, sizeof(fk_cdouble) * nfreq, hipMemcpyDeviceToHost), "copy");
+        if (a.acht_n1)
+            hip_check(hipMemcpy(acht_n1.data(), o_n1, sizeof(fk_cdouble) * nb, hipMemcpyDeviceToHost), "copy");
+        if (a.achstemp)
+            hip_check(hipMemcpy(achstemp.data(), o_achs, sizeof(fk_cdouble), hipMemcpyDeviceToHost), "copy");
+    }
+
+    for (int iw = 0; iw < nfreq; ++iw)
+        std::cout << "achtemp[" << iw << "] = " << std::setprecision(15)
+                  << std::complex<double>(achtemp[iw].re, achtemp[iw].im) << std::endl;
+    for (int iw = 0; iw < nfreq; ++iw)
+        std::cout << "asxtemp[" << iw << "] = " << std::setprecision(15)
+                  << std::complex<double>(asxtemp[iw].re, asxtemp[iw].im) << std::endl;
+    if (a.acht_n1)
+        for (int n1 = 0; n1 < nb; ++n1)
+            std::cout << "acht_n1_loc[" << n1 << "] = " << std::setprecision(15)
+                      << std::complex<double>(acht_n1[n1].re, acht_n1[n1].im) << std::endl;
+    if (a.achstemp)
+        std::cout << "achstemp = " << std::setprecision(15) << std::complex<double>(achstemp[0].re, achstemp[0].im)
+                  << std::endl;
+    const double updates = (double)nb * ngpown * ncouls * nfreq;
+    std::cout << std::setprecision(6) << "Kernel time = " << kms_sum / a.repeat << " ms per call ("
+              << updates / (kms_sum / a.repeat * 1e-3) << " updates/s, " << a.repeat << " call"
+              << (a.repeat > 1 ? "s" : "") << ", " << (opt.precision == FK_PREC_MIXED ? "mixed" : "fp64") << ")"
+              << std::endl;
+    std::cout << "********** Chrono Time Taken **********= " << elapsed.count() << " secs" << std::endl;
+
+    if (d_out) (void)hipFree(d_out);
+}
+
+// The inputs of this run as an instance file (device arrays copied back).
+void dump_instance(const std::string &path, const fk_gpp_problem &p, bool c64, bool on_dev)
+{
+    fk_gpp_problem q = p;
+    const size_t esz = c64 ? sizeof(fk_cfloat) : sizeof(fk_cdouble);
+    const size_t nbc = (size_t)p.number_bands * p.ncouls, gc = (size_t)p.ngpown * p.ncouls;
+    std::vector<char> m, n, e, w;
+    std::vector<double> v;
+    if (on_dev) {
+        m.resize(nbc * esz); n.resize(nbc * esz); e.resize(gc * esz); w.resize(gc * esz); v.resize(p.ncouls);
+        hip_check(hipMemcpy(m.data(), p.aqsmtemp, m.size(), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(n.data(), p.aqsntemp, n.size(), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(e.data(), p.I_eps_array, e.size(), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(w.data(), p.wtilde_array, w.size(), hipMemcpyDeviceToHost), "dump copy");
+        hip_check(hipMemcpy(v.data(), p.vcoul, sizeof(double) * v.size(), hipMemcpyDeviceToHost), "dump copy");
+        q.aqsmtemp = m.data(); q.aqsntemp = n.data(); q.I_eps_array = e.data(); q.wtilde_array = w.data();
+        q.vcoul = v.data();
+    }
+    const int rc = fk_gpp_instance_write(path.c_str(), &q, c64 ? 1 : 0);
+    if (rc != FK_OK) fail("fk_gpp_instance_write", rc);
+    std::cout << "Inputs written to " << path << std::endl;
+}
+
+// --input: the problem of an instance file, host memory.
+int run_input(const Args &a)
+{
+    fk_instance *inst = nullptr;
+    int rc = fk_instance_open(a.input.c_str(), &inst);
+    if (rc != FK_OK) fail("fk_instance_open", rc);
+    const fk_gpp_problem *p = fk_instance_gpp(inst);
+    if (!p) {
+        std::cerr << "gppKer: " << a.input << " holds an FF problem, not a GPP one" << std::endl;
+        fk_instance_close(inst);
+        return 2;
+    }
+    if (a.device >= 0) hip_check(hipSetDevice(a.device), "hipSetDevice");
+    const bool c64 = fk_instance_complex_bytes(inst) == (int32_t)sizeof(fk_cfloat);
+    std::cout << "Instance = " << a.input << " (" << (c64 ? "complex64" : "complex128") << ")" << std::endl;
+    std::cout << "number_bands = " << p->number_bands << "\t nvband = " << p->nvband << "\t ncouls = " << p->ncouls
+              << "\t ngpown = " << p->ngpown << "\t nend = " << p->nfreq << "\t limitone = " << p->limitone
+              << "\t limittwo = " << p->limittwo << std::endl;
+    fk_options opt = fk_default_options();
+    opt.memory = FK_MEM_HOST;
+    opt.precision = c64 ? FK_PREC_MIXED : FK_PREC_FP64;
+    opt.mask = a.unmasked ? 0 : 1;
+    opt.device = a.device;
+    run_and_print(*p, opt, a);
+    fk_instance_close(inst);
+    return 0;
 }
 
 }  // namespace
@@ -108,6 +230,7 @@ int main(int argc, char **argv)
         usage();
         return 0;
     }
+    if (!a.input.empty()) return run_input(a);
     if (a.nb < 1 || a.nv < 0 || a.nv > a.nb || a.ncouls < 1 || a.npg < 1 || a.npg > a.ncouls) {
         std::cerr << "gppKer: need number_bands >= 1, 0 <= number_valence_bands <= number_bands, "
                      "1 <= nodes_per_mpi_group <= number_plane_waves" << std::endl;
@@ -202,60 +325,10 @@ int main(int argc, char **argv)
     p.inv_igp_index = inv.data(); p.indinv = indinv.data();
     p.to1 = to1; p.limittwo = limittwo; p.limitone = limitone; p.occ = 1.0;
 
-    // outputs: host for FK_MEM_HOST, device (copied back) for FK_MEM_DEVICE
-    std::vector<fk_cdouble> achtemp(nfreq), asxtemp(nfreq), acht_n1(a.acht_n1 ? nb : 0), achstemp(1);
-    fk_cdouble *o_ach = achtemp.data(), *o_asx = asxtemp.data(), *o_n1 = a.acht_n1 ? acht_n1.data() : nullptr;
-    fk_cdouble *o_achs = a.achstemp ? achstemp.data() : nullptr;
-    fk_cdouble *d_out = nullptr;
-    if (a.random) {
-        hip_check(hipMalloc((void **)&d_out, sizeof(fk_cdouble) * (2 * nfreq + nb + 1)), "hipMalloc outputs");
-        o_ach = d_out;
-        o_asx = d_out + nfreq;
-        o_n1 = a.acht_n1 ? d_out + 2 * nfreq : nullptr;
-        o_achs = a.achstemp ? d_out + 2 * nfreq + nb : nullptr;
-    }
-    fk_gpp_outputs outs;
-    outs.achtemp = o_ach; outs.asxtemp = o_asx; outs.acht_n1_loc = o_n1; outs.achstemp = o_achs;
-    float kms = 0.f;
-    double kms_sum = 0.0;
-    opt.kernel_ms = &kms;
-    const auto t0 = std::chrono::high_resolution_clock::now();
-    for (int r = 0; r < a.repeat; ++r) {
-        int rc = gpp_kernel_ex(&p, &opt, &outs);
-        if (rc != FK_OK) fail("gpp_kernel", rc);
-        kms_sum += kms;
-    }
-    if (a.random) hip_check(hipDeviceSynchronize(), "gpp_kernel");
-    const std::chrono::duration<double> elapsed = std::chrono::high_resolution_clock::now() - t0;
-    if (a.random) {
-        hip_check(hipMemcpy(achtemp.data(), o_ach, sizeof(fk_cdouble) * nfreq, hipMemcpyDeviceToHost), "copy");
-        hip_check(hipMemcpy(asxtemp.data(), o_asx, sizeof(fk_cdouble) * nfreq, hipMemcpyDeviceToHost), "copy");
-        if (a.acht_n1)
-            hip_check(hipMemcpy(acht_n1.data(), o_n1, sizeof(fk_cdouble) * nb, hipMemcpyDeviceToHost), "copy");
-        if (a.achstemp)
-            hip_check(hipMemcpy(achstemp.data(), o_achs, sizeof(fk_cdouble), hipMemcpyDeviceToHost), "copy");
-    }
+    if (!a.dump.empty()) dump_instance(a.dump, p, a.mixed, opt.memory == FK_MEM_DEVICE);
+    run_and_print(p, opt, a);
 
-    for (int iw = 0; iw < nfreq; ++iw)
-        std::cout << "achtemp[" << iw << "] = " << std::setprecision(15)
-                  << std::complex<double>(achtemp[iw].re, achtemp[iw].im) << std::endl;
-    for (int iw = 0; iw < nfreq; ++iw)
-        std::cout << "asxtemp[" << iw << "] = " << std::setprecision(15)
-                  << std::complex<double>(asxtemp[iw].re, asxtemp[iw].im) << std::endl;
-    if (a.acht_n1)
-        for (int n1 = 0; n1 < nb; ++n1)
-            std::cout << "acht_n1_loc[" << n1 << "] = " << std::setprecision(15)
-                      << std::complex<double>(acht_n1[n1].re, acht_n1[n1].im) << std::endl;
-    if (a.achstemp)
-        std::cout << "achstemp = " << std::setprecision(15) << std::complex<double>(achstemp[0].re, achstemp[0].im)
-                  << std::endl;
-    const double updates = (double)nb * ngpown * ncouls * nfreq;
-    std::cout << std::setprecision(6) << "Kernel time = " << kms_sum / a.repeat << " ms per call ("
-              << updates / (kms_sum / a.repeat * 1e-3) << " updates/s, " << a.repeat << " call"
-              << (a.repeat > 1 ? "s" : "") << ", " << (a.mixed ? "mixed" : "fp64") << ")" << std::endl;
-    std::cout << "********** Chrono Time Taken **********= " << elapsed.count() << " secs" << std::endl;
-
-    for (void *ptr : {d_m, d_n, d_e, d_w, (void *)d_v, (void *)d_out})
+    for (void *ptr : {d_m, d_n, d_e, d_w, (void *)d_v})
         if (ptr) (void)hipFree(ptr);
     return 0;
 }

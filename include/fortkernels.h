@@ -307,6 +307,51 @@ int ff_kernel_full(const fk_ff_full_problem *problem, const fk_options *options,
 size_t ff_kernel_full_workspace_size(const fk_ff_full_problem *problem, const fk_options *options);
 
 /* ------------------------------------------------------------------------
+ * Instance files: one GPP or FF problem in one binary file, for inputs dumped
+ * from a BerkeleyGW run and for lists of batched problems.  No counterpart in
+ * the reference, whose programs synthesise their inputs
+ * (BGW-Kernels/gppKer.cpp:274-291, FFKer_cori/ffKernel.cpp:395-450).
+ *
+ * Layout (little-endian): a 256-byte header -- char magic[8] = "FKINST01",
+ * int32 kind (1 GPP, 2 FF), int32 complex_bytes (16, or 8 for a complex64
+ * GPP problem), int32 sizes[8], double scalars[8], int64 offsets[16] (file
+ * offsets of the arrays, 64-byte aligned), zero padding -- then the arrays
+ * in the ABI's C layouts (a Fortran column-major dump of the same arrays is
+ * already in that order):
+ *   GPP sizes  number_bands, nvband, ncouls, ngpown, nfreq
+ *       scalars to1, limittwo, limitone, occ
+ *       arrays  wx[nfreq] f64, inv_igp_index[ngpown] i32, indinv[ncouls+1]
+ *               i32, vcoul[ncouls] f64, aqsmtemp, aqsntemp [nb][ncouls],
+ *               I_eps_array, wtilde_array [ngpown][ncouls]
+ *   FF  sizes  number_bands, nvband, ncouls, ngpown, nfreq, nfreqeval,
+ *              achs_freq
+ *       scalars pref_zb, freqevalmin, freqevalstep, occ
+ *       arrays  ekq[nb], dFreqGrid[nfreq], pref[nfreq] f64, dFreqBrd[nfreq]
+ *               c128, inv_igp_index, indinv, vcoul, aqsmtemp, aqsntemp,
+ *               I_epsR_array, I_epsA_array [nfreq][ngpown][ncouls] c128
+ * fk_instance_open maps the file read-only and validates it; the problem it
+ * returns points into the mapping (host memory, FK_MEM_HOST) until
+ * fk_instance_close.  An instance list is a text file with one instance
+ * path per line (relative paths are relative to the list; '#' starts a
+ * comment line).
+ * ------------------------------------------------------------------------ */
+typedef enum { FK_INSTANCE_GPP = 1, FK_INSTANCE_FF = 2 } fk_instance_kind_t;
+typedef struct fk_instance fk_instance;
+
+/* complex64 != 0: the problem's complex arrays are fk_cfloat */
+int fk_gpp_instance_write(const char *path, const fk_gpp_problem *problem, int32_t complex64);
+int fk_ff_instance_write(const char *path, const fk_ff_full_problem *problem);
+int fk_instance_open(const char *path, fk_instance **instance);
+int fk_instance_kind(const fk_instance *instance);
+int32_t fk_instance_complex_bytes(const fk_instance *instance);
+const fk_gpp_problem *fk_instance_gpp(const fk_instance *instance);   /* NULL unless a GPP file */
+const fk_ff_full_problem *fk_instance_ff(const fk_instance *instance); /* NULL unless an FF file */
+void fk_instance_close(fk_instance *instance);
+/* paths: count malloc'd strings, released by fk_instance_list_free */
+int fk_instance_list_read(const char *path, char ***paths, int32_t *count);
+void fk_instance_list_free(char **paths, int32_t count);
+
+/* ------------------------------------------------------------------------
  * Utilities
  * ------------------------------------------------------------------------ */
 
