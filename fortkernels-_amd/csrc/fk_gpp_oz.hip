@@ -417,7 +417,7 @@ struct PoleArgs {
     PoleParams pp;
 };
 
-template <int NWC>
+template <int NWC, bool ACHS>
 __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
 {
     constexpr int NWAVES = 4;
@@ -431,8 +431,7 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
     const double2 *brow = a.bmat + (size_t)m * a.ldn;
     const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
     // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:
-    // 192-230), taken with the first frequency pass when asked for
-    const bool want_achs = a.achs_part != nullptr;
+    // 192-230), taken with the first frequency pass in the ACHS form
     double2 sachs = make_double2(0.0, 0.0);
     for (int w0 = 0; w0 < a.pp.nw; w0 += NWC) {
         double2 sach[NWC], sasx[NWC];
@@ -446,9 +445,11 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
                 double2 Bo = make_double2(0.0, 0.0);
                 for (int n1 = 0; n1 < a.nv; ++n1)
                     Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
-                const double2 ie = widen(erow[ig]);
-                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), ie, B, Bo, want_ssx, sach, sasx);
-                if (want_achs && w0 == 0) sachs = csub(sachs, cmul(B, ie));
+                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), widen(erow[ig]), B, Bo, want_ssx, sach,
+                                        sasx);
+                if constexpr (ACHS) {
+                    if (w0 == 0) sachs = csub(sachs, cmul(B, widen(erow[ig])));
+                }
             }
         }
 #pragma unroll
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
         }
         __syncthreads();
     }
-    if (want_achs) {
+    if constexpr (ACHS) {
         const double v0 = wave_sum(sachs.x), v1 = wave_sum(sachs.y);
         if (lane == 0) {
             red[wave][0] = v0;
@@ -493,6 +494,17 @@ __global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
 }
 
 constexpr int kPoleChunks = 4;   // column chunks per row of the pole kernel
+
+template <bool ACHS>
+void launch_pole(int nfreq, int blocks, hipStream_t s, const PoleArgs &pa)
+{
+    switch (std::min(nfreq, 4)) {
+    case 1: hipLaunchKernelGGL((gpp_oz_pole_kernel<1, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
+    case 2: hipLaunchKernelGGL((gpp_oz_pole_kernel<2, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
+    case 3: hipLaunchKernelGGL((gpp_oz_pole_kernel<3, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
+    default: hipLaunchKernelGGL((gpp_oz_pole_kernel<4, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
+    }
+}
 
 }  // namespace
 
@@ -561,12 +573,8 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     pa.chunks = kPoleChunks;
     pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
     const int blocks = p->ngpown * kPoleChunks;
-    switch (std::min(p->nfreq, 4)) {
-    case 1: hipLaunchKernelGGL(gpp_oz_pole_kernel<1>, dim3(blocks), dim3(256), 0, s, pa); break;
-    case 2: hipLaunchKernelGGL(gpp_oz_pole_kernel<2>, dim3(blocks), dim3(256), 0, s, pa); break;
-    case 3: hipLaunchKernelGGL(gpp_oz_pole_kernel<3>, dim3(blocks), dim3(256), 0, s, pa); break;
-    default: hipLaunchKernelGGL(gpp_oz_pole_kernel<4>, dim3(blocks), dim3(256), 0, s, pa); break;
-    }
+    if (pa.achs_part) launch_pole<true>(p->nfreq, blocks, s, pa);
+    else launch_pole<false>(p->nfreq, blocks, s, pa);
     FK_HIP(hipGetLastError());
     FK_TRY(gpp_reduce_launch(pa.partials, blocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
     if (d_achs) FK_TRY(gpp_sum_launch(pa.achs_part, blocks, d_achs, s));
