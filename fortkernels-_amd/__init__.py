@@ -36,7 +36,8 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size",
            "gpp_kernel_ex", "fk_gpp_instance_write", "fk_ff_instance_write", "fk_instance_open",
            "fk_instance_kind", "fk_instance_complex_bytes", "fk_instance_gpp", "fk_instance_ff", "fk_instance_close",
-           "fk_instance_list_read", "fk_instance_list_free"]
+           "fk_instance_list_read", "fk_instance_list_free", "ff_batch_create", "ff_batch_run", "ff_batch_config",
+           "ff_batch_workspace_bytes", "ff_batch_destroy"]
 
 
 class FkError(RuntimeError):
@@ -136,6 +137,18 @@ def _load():
                                         ctypes.POINTER(I32)]
     L.fk_instance_list_free.restype = None
     L.fk_instance_list_free.argtypes = [ctypes.POINTER(ctypes.c_char_p), I32]
+    L.ff_batch_create.restype = ctypes.c_int
+    L.ff_batch_create.argtypes = [I32, ctypes.POINTER(ctypes.POINTER(FfProblem)), ctypes.POINTER(Options),
+                                  ctypes.POINTER(P)]
+    L.ff_batch_run.restype = ctypes.c_int
+    L.ff_batch_run.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P), P,
+                               ctypes.POINTER(ctypes.c_float)]
+    L.ff_batch_config.restype = ctypes.c_int
+    L.ff_batch_config.argtypes = [P]
+    L.ff_batch_workspace_bytes.restype = SZ
+    L.ff_batch_workspace_bytes.argtypes = [P]
+    L.ff_batch_destroy.restype = None
+    L.ff_batch_destroy.argtypes = [P]
     L.gpp_kernel_ex.restype = ctypes.c_int
     L.gpp_kernel_ex.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), ctypes.POINTER(GppOutputs)]
     L.ff_kernel_full.restype = ctypes.c_int
@@ -292,6 +305,67 @@ def gpp_problem(p, **kw):
 # --------------------------------------------------------------------------
 # batched GPP
 # --------------------------------------------------------------------------
+class FfBatch:
+    """A batch of FF contractions on the device (ff_batch_create).
+
+    problems: sequence of dicts with the ff_kernel arguments: nb, ncouls,
+    ngpown, nfreq, aqsm, aqsn, I_eps, vcoul (torch device tensors),
+    inv_igp_index, indinv and optionally achs_freq.  config 0 lets the
+    library time the precision's configurations on the batch."""
+
+    def __init__(self, problems, precision="fp64", config=0):
+        self._keep = []
+        structs = []
+        for p in problems:
+            inv = np.ascontiguousarray(p["inv_igp_index"], np.int32)
+            ind = np.ascontiguousarray(p["indinv"], np.int32)
+            self._keep += [inv, ind, p]
+            structs.append(FfProblem(int(p["nb"]), int(p["ncouls"]), int(p["ngpown"]), int(p["nfreq"]),
+                                     _ptr(p["aqsm"]), _ptr(p["aqsn"]), _ptr(p["I_eps"]), _ptr(p["vcoul"]),
+                                     inv.ctypes.data, ind.ctypes.data, int(p.get("achs_freq", 0))))
+        self.n = len(structs)
+        self.shapes = [(s.number_bands, s.nfreq) for s in structs]
+        self._structs = structs
+        arr = (ctypes.POINTER(FfProblem) * self.n)(*[ctypes.pointer(s) for s in structs])
+        opt = default_options()
+        opt.memory = FK_MEM_DEVICE
+        opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+        opt.config = int(config)
+        h = ctypes.c_void_p()
+        _check(lib.ff_batch_create(self.n, arr, ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def config(self):
+        return lib.ff_batch_config(self._h)
+
+    @property
+    def workspace_bytes(self):
+        return int(lib.ff_batch_workspace_bytes(self._h))
+
+    def run(self, stream=None, time_it=False, want_schDv=True, want_achs=True):
+        """Returns ([schDt_i], [schDv_i] | None, [achsDtemp_i] | None, ms | None)
+        as torch device tensors."""
+        import torch
+        mk = lambda *s: torch.empty(*s, dtype=torch.complex128, device="cuda")  # noqa: E731
+        t = [mk(nb, nf) for nb, nf in self.shapes]
+        v = [mk(nb, nf) for nb, nf in self.shapes] if want_schDv else None
+        a = [mk(1) for _ in self.shapes] if want_achs else None
+        arr = lambda xs: (ctypes.c_void_p * self.n)(*[x.data_ptr() for x in xs]) if xs is not None else None  # noqa: E731
+        ms = ctypes.c_float(-1.0)
+        _check(lib.ff_batch_run(self._h, arr(t), arr(v), arr(a), _stream_handle(stream),
+                                ctypes.byref(ms) if time_it else None))
+        return t, v, a, (float(ms.value) if time_it else None)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ff_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class GppBatch:
     """A batch of GPP problems on the device (gpp_batch_create).
 

@@ -298,6 +298,8 @@ void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
 
 }  // namespace
 
+int ff_fp64_configs() { return kNumFfCfgs; }
+
 FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
 {
     FfLayout L;
@@ -382,14 +384,15 @@ static int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X
 
 int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
               const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *X,
-              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s)
+              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s,
+              bool upload_cols)
 {
     const int nb = p->number_bands, C = p->ncouls, G = p->ngpown, nf = p->nfreq;
     double2 *cf = static_cast<double2 *>(ws.at(L.off_cf));
     double *vg = static_cast<double *>(ws.at(L.off_vg));
     int *igp = static_cast<int *>(ws.at(L.off_igp));
     double2 *part = static_cast<double2 *>(ws.at(L.off_part));
-    FK_HIP(hipMemcpyAsync(igp, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, s));
+    if (upload_cols) FK_HIP(hipMemcpyAsync(igp, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, s));
     {
         int64_t n = (int64_t)nb * G;
         int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);

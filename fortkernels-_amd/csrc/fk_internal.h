@@ -129,7 +129,9 @@ int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, co
 // runs the coefficient kernel, the GEMM (FP64 MFMA, or split BF16 for
 // FK_PREC_MIXED) and the fixed-order reduction into schDt / schDv (either
 // may be NULL) and, if achs is non-NULL, achsDtemp at p->achs_freq.
-// options->kernel_ms times the GEMM kernel.
+// options->kernel_ms times the GEMM kernel.  upload_cols = false: the column
+// map is already in the layout's igp slot (a caller that captures the
+// launches into a graph uploads it first).
 // fk_options.config of the FP64 FF GEMM with 16-row tiles (for few bands)
 constexpr int kFfConfigFewBands = 3;
 struct FfLayout {
@@ -138,10 +140,12 @@ struct FfLayout {
     FfSplitLayout sp;
     size_t off_cf, off_vg, off_igp, off_part, off_achs;
 };
+int ff_fp64_configs();   // FP64 tile configurations (fk_options.config 1..this)
 FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws);
 int ff_check(const fk_ff_problem *p, const fk_options *o);
 int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Scratch &ws,
               const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *X,
-              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s);
+              const double *vcoul, double2 *schDt, double2 *schDv, double2 *achs, double vscale, hipStream_t s,
+              bool upload_cols = true);
 
 }  // namespace fk

@@ -235,6 +235,34 @@ int ff_kernel(const fk_ff_problem *problem, const fk_options *options,
 size_t ff_kernel_workspace_size(const fk_ff_problem *problem, const fk_options *options);
 
 /* ------------------------------------------------------------------------
+ * Batched FF: many ff_kernel problems (uneven sizes) in one graph launch.
+ * No counterpart in the reference, whose program contracts one problem per
+ * process (ffkernel.deslippe.best.f90.back1:402-418, FFKer_cori/ffKernel.cpp:
+ * 107-131); each problem gets exactly what ff_kernel computes for it.
+ * ff_batch_create lays out one device allocation for every problem's
+ * workspace and results, picks the configuration (options->config > 0, or
+ * with 0 the fastest of the precision's configurations timed on this batch
+ * -- create then runs the batch twice per configuration, so the inputs must
+ * be valid) and captures every problem's launches into one graph.  Inputs
+ * are device memory (FK_MEM_DEVICE), read again at every run; the arrays
+ * must outlive the batch.  ff_batch_run launches the graph on `stream`, then
+ * copies problem i's results to schDt[i] ([nb_i][nfreq_i]), schDv[i] and
+ * achsDtemp[i] (device pointers; schDv / achsDtemp, or entries of them, may
+ * be NULL); elapsed_ms (optional) receives the graph's duration and
+ * synchronises.  Calls that share one batch must be serialised.
+ * ------------------------------------------------------------------------ */
+typedef struct fk_ff_batch fk_ff_batch;
+
+int ff_batch_create(int32_t n, const fk_ff_problem *const *problems, const fk_options *options,
+                    fk_ff_batch **batch);
+int ff_batch_run(fk_ff_batch *batch, fk_cdouble *const *schDt, fk_cdouble *const *schDv,
+                 fk_cdouble *const *achsDtemp, void *stream, float *elapsed_ms);
+/* Configuration the batch uses (as fk_options.config). */
+int ff_batch_config(const fk_ff_batch *batch);
+size_t ff_batch_workspace_bytes(const fk_ff_batch *batch);
+void ff_batch_destroy(fk_ff_batch *batch);
+
+/* ------------------------------------------------------------------------
  * Whole FF self-energy evaluation: the Fortran program
  * ffkernel.deslippe.best.f90:251-631, with schdt_matrix from the contraction
  * of ffkernel.deslippe.best.f90.back1:402-418.  For the evaluation axis

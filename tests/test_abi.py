@@ -186,3 +186,21 @@ def test_ff_full_workspace_and_checks(fk, fko):
     with pytest.raises(fk.FkError) as e:
         fk.ff_full_problem(p)
     assert e.value.code == fk.FK_ERR_ARG and "increasing" in fk.last_error()
+
+
+def test_ff_batch_rejects_host_memory_and_bad_config(fk, fko):
+    """Argument checks of ff_batch_create run before any device work."""
+    import ctypes
+    p = fko.ff_random(4, 32, 8, 2, seed=1)
+    s = fk.FfProblem(p.nb, p.ncouls, p.ngpown, p.nfreq, 1, 1, 1, 1, p.inv_igp_index.ctypes.data,
+                     p.indinv.ctypes.data, 0)
+    arr = (ctypes.POINTER(fk.FfProblem) * 1)(ctypes.pointer(s))
+    h = ctypes.c_void_p()
+    opt = fk.default_options()          # host memory
+    assert fk.lib.ff_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_UNSUPPORTED
+    assert "FK_MEM_DEVICE" in fk.last_error()
+    opt.memory = fk.FK_MEM_DEVICE
+    assert fk.lib.ff_batch_create(0, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
+    opt.config = 9
+    assert fk.lib.ff_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
+    assert not h.value
