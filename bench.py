@@ -418,7 +418,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ff", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--cpu-bands", type=int, default=128)
+    ap.add_argument("--cpu-bands", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (HBM counter calibration)")
