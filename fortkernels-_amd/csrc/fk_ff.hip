@@ -37,6 +37,7 @@ struct FfArgs {
     int nb, C, G, nfreq, nks;
     int tiles_m, tiles_n, mtiles;   // tiles_n = nfreq * mtiles
     int stagger_lo, stagger_hi, stagger_ticks;
+    int ksplit;             // ff_fewband_kernel: ig ranges per column block
 };
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
@@ -226,6 +227,138 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
     }
 }
 
+// Few bands (at most 16 rows per tile): the contraction is bound by reading
+// X once, so X and aqsn pass through LDS in chunks of 16 ig, loaded with
+// full-width coalesced reads (one 256-byte row segment per 16 lanes), double
+// buffered, and the MFMA fragments come from LDS (row pitch 17 double2, so
+// the 16 rows of one 16-lane read pass fall in distinct banks).  Workgroup:
+// 16 bands x 64 columns of one frequency, four waves of one 16x16 block.
+// The ig range may be split over ksplit workgroups (more workgroups, even
+// load over the CUs); each writes its own partial (column tile index
+// mt = column block * ksplit + part), which the reduction sums.
+// Same partials as ff_fused_kernel.
+constexpr int kFbBM = 16, kFbBN = 64, kFbKC = 16, kFbPitch = kFbKC + 1;
+
+__global__ __launch_bounds__(256, 2) void ff_fewband_kernel(FfArgs a)
+{
+    __shared__ double2 xs[2][kFbBN][kFbPitch];
+    __shared__ double2 as[2][kFbBM][kFbPitch];
+    __shared__ double2 red[4][2 * kFbBM];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kl = lane >> 4, cl = lane & 15;
+    const int tm = blockIdx.x % a.tiles_m, tn = blockIdx.x / a.tiles_m;
+    const int f = tn / a.mtiles, mt = tn % a.mtiles;
+    const int kpart = mt % a.ksplit;
+    const int rowbase = tm * kFbBM, colbase = (mt / a.ksplit) * kFbBN;
+    const int nall = a.nks / 4;
+    const int c0 = (int)((int64_t)kpart * nall / a.ksplit), c1 = (int)((int64_t)(kpart + 1) * nall / a.ksplit);
+    const int nchunks = c1 - c0;
+    // this thread's global loads of a chunk: X element e = 256 k + tid
+    // (row e / 16, ig e % 16), k = 0..3; aqsn element tid (row tid / 16)
+    const double2 *xrow[4];
+    int xr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int e = 256 * k + tid;
+        xr[k] = e >> 4;
+        xrow[k] = a.X + ((size_t)f * a.G + min(colbase + xr[k], a.G - 1)) * a.C;
+    }
+    const int ar = tid >> 4, ai = tid & 15;
+    const double2 *arow = a.aqsn + (size_t)min(rowbase + ar, a.nb - 1) * a.C;
+    // two chunks in flight in registers (sets R0, R1), one in each LDS buffer
+    struct Regs { double2 x[4], a; };
+    auto fetch = [&](int c, Regs &r) {
+        const int ig = (c0 + c) * kFbKC + ai;
+        const bool ok = ig < a.C;
+        const int igc = min(ig, a.C - 1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double2 v = xrow[k][igc];
+            r.x[k] = ok ? v : make_double2(0.0, 0.0);
+        }
+        const double2 v = arow[igc];
+        r.a = ok ? v : make_double2(0.0, 0.0);
+    };
+    auto stash = [&](int b, const Regs &r) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) xs[b][xr[k]][ai] = r.x[k];
+        as[b][ar][ai] = r.a;
+    };
+    d4 acc_re = d4{0, 0, 0, 0}, acc_im = d4{0, 0, 0, 0};
+    auto compute = [&](int b) {
+#pragma unroll
+        for (int s = 0; s < kFbKC / 4; ++s) {
+            const double2 av = as[b][cl][4 * s + kl];
+            const double2 bv = xs[b][w * 16 + cl][4 * s + kl];
+            acc_re = mfma(av.x, bv.x, acc_re);
+            acc_im = mfma(av.x, bv.y, acc_im);
+            acc_re = mfma_nega(av.y, bv.y, acc_re);
+            acc_im = mfma(av.y, bv.x, acc_im);
+        }
+    };
+    Regs r0, r1;
+    if (nchunks < 1) {   // an empty ig range still writes its (zero) partial
+        r0.a = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r0.x[k] = make_double2(0.0, 0.0);
+    } else {
+        fetch(0, r0);
+    }
+    if (1 < nchunks) fetch(1, r1);
+    stash(0, r0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; c += 2) {
+        // chunk c in buffer 0, chunk c+1 in r1
+        if (c + 2 < nchunks) fetch(c + 2, r0);
+        compute(0);
+        if (c + 1 < nchunks) stash(1, r1);
+        __syncthreads();
+        if (c + 1 >= nchunks) break;
+        // chunk c+1 in buffer 1, chunk c+2 in r0
+        if (c + 3 < nchunks) fetch(c + 3, r1);
+        compute(1);
+        if (c + 2 < nchunks) stash(0, r0);
+        __syncthreads();
+    }
+    // contract each band with cf (and vg cf) over the wave's 16 columns:
+    // register r of lane l holds T[band 4 r + l / 16][column l % 16]
+    const int col = colbase + w * 16 + cl;
+    const bool cok = col < a.G;
+    const double vgc = a.vg[min(col, a.G - 1)];
+    double2 rt[4], rv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = rowbase + 4 * r + kl;
+        const double2 cfv = a.cf[(size_t)min(row, a.nb - 1) * a.G + min(col, a.G - 1)];
+        const double2 p = cmul(make_double2(acc_re[r], acc_im[r]), cfv);
+        rt[r] = cok ? p : make_double2(0.0, 0.0);
+        rv[r] = cok ? make_double2(p.x * vgc, p.y * vgc) : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            rt[r].x += __shfl_xor(rt[r].x, off, 64);
+            rt[r].y += __shfl_xor(rt[r].y, off, 64);
+            rv[r].x += __shfl_xor(rv[r].x, off, 64);
+            rv[r].y += __shfl_xor(rv[r].y, off, 64);
+        }
+    }
+    if (cl == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            red[w][2 * (4 * r + kl)] = rt[r];
+            red[w][2 * (4 * r + kl) + 1] = rv[r];
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * kFbBM) {
+        const int lr = tid >> 1, which = tid & 1;
+        double2 sum = red[0][tid];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) sum = cadd(sum, red[q][tid]);
+        const int n1 = rowbase + lr;
+        if (n1 < a.nb) a.partials[(((size_t)f * a.mtiles + mt) * a.nb + n1) * 2 + which] = sum;
+    }
+}
+
 // schDt[n1][f], schDv[n1][f]: sum over the column tiles in order.  Thread
 // (f, n1), n1 fastest, so a warp's partials are contiguous; the row sums of
 // frequency achs_freq (unscaled schDv) also go to achs_rows[n1].
@@ -279,10 +412,11 @@ struct FfCfg { int bm, bn, p, minw; };
 //   0: 32x32 waves, 2x2 waves (64x64 tile), ring 3, 2 workgroups/CU
 //   1: 32x64 waves, 2x2 waves (64x128 tile), ring 3, 1 workgroup/CU (default:
 //      15.4 ms against 16.2 ms for id 0 at 256/4096/4096/32 on MI355X)
-//   2: 16x32 waves, 1x4 waves (16x128 tile), ring 3, 2 workgroups/CU: for
-//      16 bands or fewer, where a 64-row tile would spend 4x the MFMA work
+//   2: ff_fewband_kernel, 16x64 tile through LDS (chunks of 16 ig = 4
+//      k-steps): for 16 bands or fewer, where the product is bound by
+//      reading X and a 64-row tile would spend 4x the MFMA work
 //      (kFfConfigFewBands)
-constexpr FfCfg kFfCfgs[] = {{64, 64, 3, 2}, {64, 128, 3, 1}, {16, 128, 3, 2}};
+constexpr FfCfg kFfCfgs[] = {{64, 64, 3, 2}, {64, 128, 3, 1}, {kFbBM, kFbBN, 4, 2}};
 static_assert(kFfConfigFewBands == 3, "kFfConfigFewBands names the 16-row tile");
 constexpr int kFfDefaultCfg = 1;
 constexpr int kNumFfCfgs = sizeof(kFfCfgs) / sizeof(kFfCfgs[0]);
@@ -291,7 +425,7 @@ void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
 {
     switch (cfg) {
     case 0: hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2>), dim3(tiles), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((ff_fused_kernel<1, 2, 3, 1, 4, 2>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(ff_fewband_kernel, dim3(tiles), dim3(256), 0, s, a); break;
     default: hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1>), dim3(tiles), dim3(256), 0, s, a); break;
     }
 }
@@ -316,6 +450,13 @@ FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
         L.tiles_m = div_up(std::max(p->number_bands, 1), c.bm);
         L.mtiles = div_up(p->ngpown, c.bn);
         L.nks = div_up(p->ncouls, 4 * c.p) * c.p;
+        if (L.cfg == kFfConfigFewBands - 1) {
+            // split the ig range until there are about four workgroups per
+            // CU, keeping at least two chunks per part
+            const int tiles = L.tiles_m * p->nfreq * L.mtiles, chunks = L.nks / 4;
+            while (tiles * L.ksplit < 1024 && L.ksplit < 8 && chunks / (2 * L.ksplit) >= 2) L.ksplit *= 2;
+            L.mtiles *= L.ksplit;
+        }
     }
     L.off_cf = ws.plan(sizeof(double2) * (size_t)std::max(p->number_bands, 1) * p->ngpown);
     L.off_vg = ws.plan(sizeof(double) * (size_t)p->ngpown);
@@ -363,9 +504,10 @@ static int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X
     a.aqsn = aqsn; a.X = X; a.cf = cf; a.vg = vg; a.partials = part;
     a.nb = nb; a.C = C; a.G = G; a.nfreq = nf; a.nks = L.nks;
     a.tiles_m = L.tiles_m; a.mtiles = L.mtiles; a.tiles_n = nf * L.mtiles;
+    a.ksplit = L.ksplit;
     const int tiles = a.tiles_m * a.tiles_n;
     a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
-    if (kFfCfgs[L.cfg].minw == 2) {
+    if (kFfCfgs[L.cfg].minw == 2 && L.cfg != kFfConfigFewBands - 1) {   // MFMA-bound tiles only
         int dev = 0, ncu = 0;
         FK_HIP(hipGetDevice(&dev));
         FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));

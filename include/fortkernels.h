@@ -226,7 +226,8 @@ typedef struct {
  * and achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:107).  schDt: [nb][nfreq];
  * schDv: [nb][nfreq] or NULL; achsDtemp: one value or NULL.
  * options->config: FK_PREC_FP64: 0 = default, 1 = 64x64 tiles, 2 = 64x128
- * tiles, 3 = 16x128 tiles (for 16 bands or fewer); FK_PREC_MIXED (split
+ * tiles, 3 = 16x64 tiles staged through LDS, the ig range split over
+ * workgroups (for 16 bands or fewer); FK_PREC_MIXED (split
  * BF16): ig chunk of 0 = 1024 (default), 1 = 256, 2 = 512, 3 = 1024 (a
  * smaller chunk: less FP32 rounding, more workgroups and partials). */
 int ff_kernel(const fk_ff_problem *problem, const fk_options *options,

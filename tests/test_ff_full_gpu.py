@@ -109,6 +109,21 @@ def test_mixed_vs_oracle(fk, fko, case):
     _check(got, ref, MIXED_RTOL, "mixed")
 
 
+def test_no_band_reads_I_epsA(fk, fko):
+    """No occupied band and every wx < 0: neither the SX nor the delta terms
+    read I_epsA, so its contraction is skipped; the sigma terms remain."""
+    p = _variety(fko, 17, 24, 0, 80, 16, 6, 5)
+    p.ekq = p.ekq + 100.0
+    ref = fko.ff_full(p)
+    assert np.max(np.abs(ref["asxDtemp"])) == 0
+    for memory in ("device", "host"):
+        got = _np(fk.ff_full_problem(_to_dev(p) if memory == "device" else p))
+        for k in ("achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp", "schDt"):
+            assert _rel(got[k], ref[k]) <= RTOL, (memory, k)
+        assert np.max(np.abs(got["asxDtemp"])) == 0
+        assert abs(got["achsDtemp"] - ref["achsDtemp"]) <= RTOL * abs(ref["achsDtemp"])
+
+
 def test_skipped_columns_and_achs_freq(fk, fko):
     p = _variety(fko, 11, 20, 6, 64, 16, 6, 8)
     inv = p.inv_igp_index.copy()
