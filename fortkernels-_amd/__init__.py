@@ -80,9 +80,11 @@ class FfFullProblem(ctypes.Structure):
                                                    "inv_igp_index", "indinv", "ekq", "dFreqGrid", "pref",
                                                    "dFreqBrd")]
                 + [(n, ctypes.c_double) for n in ("pref_zb", "freqevalmin", "freqevalstep", "occ")]
-                + [("achs_freq", ctypes.c_int32)])
+                + [("achs_freq", ctypes.c_int32), ("semantics", ctypes.c_int32)])
 
 
+# fk_ff_semantics
+FK_FF_FORTRAN, FK_FF_CPP_PORT, FK_FF_CPP_PORT_ZERO_SCHDT = 0, 1, 2
 FF_FULL_OUTPUTS = ("asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp")
 
 
@@ -487,14 +489,15 @@ def ff_problem(p, **kw):
 def _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, host, scal):
     return FfFullProblem(int(nb), int(nv), int(ncouls), int(ngpown), int(nfreq), int(nfreqeval), _ptr(aqsm),
                          _ptr(aqsn), _ptr(I_epsR), _ptr(I_epsA), _ptr(vcoul), *[h.ctypes.data for h in host],
-                         *[float(x) for x in scal[:4]], int(scal[4]))
+                         *[float(x) for x in scal[:4]], int(scal[4]), int(scal[5]) if len(scal) > 5 else 0)
 
 
 def ff_kernel_full(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, inv_igp_index,
                    indinv, ekq, dFreqGrid, pref, dFreqBrd, *, pref_zb=0.5 / 3.14, freqevalmin=0.0,
-                   freqevalstep=0.5, occ=1.0, achs_freq=0, stream=None, workspace=None, config=0,
-                   time_kernel=False, want_schDt=True, device=-1, precision="fp64"):
-    """Run ff_kernel_full (the whole FF evaluation of the Fortran program).
+                   freqevalstep=0.5, occ=1.0, achs_freq=0, semantics=FK_FF_FORTRAN, stream=None, workspace=None,
+                   config=0, time_kernel=False, want_schDt=True, device=-1, precision="fp64"):
+    """Run ff_kernel_full (the whole FF evaluation of the Fortran program, or
+    with semantics FK_FF_CPP_PORT[_ZERO_SCHDT] that of the C++ port).
     Bulk arrays are numpy (host) or torch device tensors; inv_igp_index,
     indinv, ekq, dFreqGrid, pref, dFreqBrd are host arrays.  Returns a dict
     with achsDtemp, asxDtemp, achDtemp, achDtemp_cor, achDtemp_corb,
@@ -505,7 +508,7 @@ def ff_kernel_full(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR,
             np.ascontiguousarray(pref, np.float64), np.ascontiguousarray(dFreqBrd, np.complex128)]
     on_dev = _is_torch(aqsn)
     prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, host,
-                           (pref_zb, freqevalmin, freqevalstep, occ, achs_freq))
+                           (pref_zb, freqevalmin, freqevalstep, occ, achs_freq, semantics))
     opt = default_options()
     opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
     opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
@@ -534,14 +537,14 @@ def ff_kernel_full(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR,
 
 
 def ff_full_workspace_size(nb, nv, ncouls, ngpown, nfreq, nfreqeval, ekq, dFreqGrid, config=0, precision="fp64",
-                           freqevalmin=0.0, freqevalstep=0.5):
+                           freqevalmin=0.0, freqevalstep=0.5, semantics=FK_FF_FORTRAN):
     """Workspace bytes of ff_kernel_full; it depends on the evaluation axis
-    (the I_epsA bands and frequencies that are read)."""
+    and the semantics (the I_epsA bands and frequencies that are read)."""
     dummy = np.zeros(1)
     host = [np.zeros(ngpown, np.int32), np.zeros(ncouls + 1, np.int32), np.ascontiguousarray(ekq, np.float64),
             np.ascontiguousarray(dFreqGrid, np.float64), np.zeros(nfreq), np.zeros(nfreq, np.complex128)]
     prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, dummy, dummy, dummy, dummy, dummy, host,
-                           (0.0, freqevalmin, freqevalstep, 1.0, 0))
+                           (0.0, freqevalmin, freqevalstep, 1.0, 0, semantics))
     opt = default_options()
     opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
     opt.config = config

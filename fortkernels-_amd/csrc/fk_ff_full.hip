@@ -22,6 +22,11 @@
 // wx is computed on the host without contraction into FMA, as the Fortran
 // evaluates it, and handed to the device, so that the interval search and the
 // sign tests see the very values the Fortran sees.
+//
+// FK_FF_CPP_PORT / FK_FF_CPP_PORT_ZERO_SCHDT evaluate instead what the C++
+// port FFKer_cori/ffKernel.cpp computes as its main calls it (:485-500): the
+// same contractions, a second form of the per-(n1, iw) kernel (cpp_terms)
+// and its own host tables and I_epsA box.
 #include "fk_complex.h"
 #include "fk_internal.h"
 
@@ -38,8 +43,14 @@ struct EpiArgs {
     const double2 *S;       // [nb][nF]   schdt_matrix (no vcoul)
     const double2 *VR;      // [nb][nF]   vcoul-weighted, I_epsR
     const double2 *VA;      // [nbA][nFA] vcoul-weighted, I_epsA, bands bA0.., frequencies fA0..
-    const double *wx;       // [nb][ne]   freqevalmin - ekq[n1] + iw freqevalstep
-    const double *wxp;      // [nb][ne]   -freqevalmin - ekq[n1] + iw freqevalstep (:509)
+    // Fortran: wx = freqevalmin - ekq[n1] + iw freqevalstep; wxp the same
+    // with -freqevalmin (:509).  C++ port: wx = freqevalmin - ekq[n1] +
+    // freqevalstep for every iw (:140, :224), wxp = freqevalmin - ekq[n1] +
+    // (iw-1) freqevalstep (:202), w2[iw] = int(freqevalmin) - ekq[nb-1] +
+    // (iw-1) int(freqevalstep) (:10-12, the last band only).
+    const double *wx;       // [nb][ne]
+    const double *wxp;      // [nb][ne]
+    const double *w2;       // [ne]
     const double *grid;     // [nF]
     const double *pref;     // [nF]
     const double2 *brd;     // [nF]
@@ -57,6 +68,16 @@ __host__ __device__ inline int find_interval(const double *grid, int nF, double 
     for (int i = 0; i < nF - 1; ++i)
         if (w >= grid[i] && w < grid[i + 1]) f = i;
     return f < 0 ? nF - 2 : f;
+}
+
+// compute_fact's search (FFKer_cori/ffKernel.cpp:34-38): the last i with
+// grid[i] < w < grid[i+1], else 0; the callers add the port's fall-backs.
+__host__ __device__ inline int find_interval_strict(const double *grid, int nF, double w)
+{
+    int f = 0;
+    for (int i = 0; i < nF - 1; ++i)
+        if (w > grid[i] && w < grid[i + 1]) f = i;
+    return f;
 }
 
 __device__ inline double clamp_ratio(double x)
@@ -78,6 +99,71 @@ __device__ inline double2 va_at(const EpiArgs &a, int n1, int f)
     return a.VA[(size_t)b * a.nFA + g];
 }
 
+// The C++ port's terms for one (n1, iw).  SX (asxDtemp_Kernel, :133-153):
+// occupied bands, VR only, compute_fact's loop 1 with the nfreq-2 fall-back.
+// Sigma 1 (achDtemp_Kernel, :196-206, written into asxDtemp by main's
+// aliasing, :495) over every frequency; ZERO_S drops it with schDt_matrix.
+// ach2Dtemp: every band adds sch2Di (:239), which holds the sigma-2 value of
+// the last band and frequency (reset per term, :191), so the last band's
+// thread adds nb copies.  Delta terms (achDtemp_cor_Kernel, :212-246 with
+// schDttt_corKernel1/2, :248-300): unoccupied bands at wx > 0 without a
+// fall-back (:55-58), occupied bands at wx <= 0 with it, scaled -0.5.
+template <bool ZERO_S>
+__device__ inline void cpp_terms(const EpiArgs &a, int n1, int iw, double2 *acc)
+{
+    const double *g = a.grid;
+    const int nF = a.nF;
+    const bool occb = n1 < a.nv;
+    const double wx = a.wx[(size_t)n1 * a.ne + iw];
+    const double2 *VR = a.VR + (size_t)n1 * nF;
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 asx = zero, cor = zero, s2 = zero;
+    if (occb) {
+        int f = find_interval_strict(g, nF, wx > 0.0 ? wx : -wx);
+        if (f == 0) f = nF - 2;
+        const double d = g[f + 1] - g[f];
+        double fa, fb;
+        if (wx > 0.0) { fa = (g[f + 1] - wx) / d; fb = (wx - g[f]) / d; }
+        else { fa = (g[f + 1] + wx) / d; fb = (-g[f] - wx) / d; }
+        asx = cscale(cadd(cscale(VR[f], fa), cscale(VR[f + 1], fb)), a.occ);
+    }
+    if (!ZERO_S) {
+        const double2 *S = a.S + (size_t)n1 * nF;
+        const double w1 = a.wxp[(size_t)n1 * a.ne + iw];
+        for (int f = 0; f < nF; ++f) {
+            const double2 br = a.brd[f];
+            const double2 coh = make_double2(g[f] - br.x, -br.y);
+            asx = csub(asx, cdiv(imul(a.pref[f], S[f]), make_double2(w1 - coh.x, -coh.y)));
+        }
+        if (n1 == a.nb - 1) {
+            const double zr = g[nF - 1], zl = g[nF - 2], base = a.w2[iw];
+            const double2 avg = cscale(cadd(S[nF - 1], S[nF - 2]), 0.5);
+            const double it = -log(clamp_ratio((base - zr) / (base - zl)));
+            s2 = cscale(csub(zero, cscale(imul(a.pref_zb, avg), it)), (double)a.nb);
+        }
+    }
+    if (wx > 0.0) {
+        if (!occb) {
+            const int f = find_interval_strict(g, nF, wx);
+            const double d = g[f + 1] - g[f];
+            const double fa = -0.5 * (g[f + 1] - wx) / d, fb = -0.5 * (wx - g[f]) / d;
+            cor = cadd(cscale(csub(VR[f], va_at(a, n1, f)), fa), cscale(csub(VR[f + 1], va_at(a, n1, f + 1)), fb));
+        }
+    } else if (occb) {
+        const double w = -wx;
+        int f = find_interval_strict(g, nF, w);
+        if (f == 0) f = nF - 2;
+        const double d = g[f + 1] - g[f];
+        const double fa = (g[f + 1] - w) / d, fb = (w - g[f]) / d;
+        const double2 s = cadd(cscale(csub(VR[f], va_at(a, n1, f)), fa),
+                               cscale(csub(VR[f + 1], va_at(a, n1, f + 1)), fb));
+        cor = cscale(s, -0.5);
+    }
+    acc[0] = asx; acc[2] = cor; acc[4] = s2;   // achDtemp, achDtemp_corb: zero
+}
+
+// MODE: fk_ff_semantics
+template <int MODE>
 __global__ __launch_bounds__(kEpiThreads) void ff_epilogue_kernel(EpiArgs a)
 {
     const int tid = threadIdx.x;
@@ -86,7 +172,9 @@ __global__ __launch_bounds__(kEpiThreads) void ff_epilogue_kernel(EpiArgs a)
     double2 acc[kTerms];
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) acc[k] = make_double2(0.0, 0.0);
-    if (n1 < a.nb) {
+    if (MODE != FK_FF_FORTRAN) {
+        if (n1 < a.nb) cpp_terms<MODE == FK_FF_CPP_PORT_ZERO_SCHDT>(a, n1, iw, acc);
+    } else if (n1 < a.nb) {
         const bool occb = n1 < a.nv;
         const double wx = a.wx[(size_t)n1 * a.ne + iw];
         const double *g = a.grid;
@@ -202,6 +290,22 @@ int full_check(const fk_ff_full_problem *p, const fk_options *o, const fk_ff_ful
                   p->number_bands, p->nvband, p->ncouls, p->ngpown, p->nfreq, p->nfreqeval);
         return FK_ERR_ARG;
     }
+    if (p->semantics < FK_FF_FORTRAN || p->semantics > FK_FF_CPP_PORT_ZERO_SCHDT) {
+        set_error("ff_kernel_full: unknown semantics %d", p->semantics);
+        return FK_ERR_ARG;
+    }
+    if (p->semantics != FK_FF_FORTRAN) {
+        // the port takes freqevalmin and freqevalstep as int (:10) and forms
+        // (iw-1) * freqevalstep in int
+        const double lim = 2147483647.0;
+        const bool ok = std::fabs(p->freqevalmin) < lim && std::fabs(p->freqevalstep) < lim &&
+                        std::fabs((double)(int)p->freqevalstep) * std::max(1, p->nfreqeval - 2) < lim;
+        if (!ok) {
+            set_error("ff_kernel_full: the C++ port's semantics need int(freqevalmin), int(freqevalstep) and "
+                      "(nfreqeval-2) * int(freqevalstep) within int32");
+            return FK_ERR_ARG;
+        }
+    }
     if (!p->I_epsA_array || !p->ekq || !p->dFreqGrid || !p->pref || !p->dFreqBrd) {
         set_error("ff_kernel_full: NULL input array");
         return FK_ERR_ARG;
@@ -232,36 +336,55 @@ struct FullPlan {
     int nb, nF, ne, nchunks;
     bool haveA;
     int bA0, nbA, fA0, nFA;
-    std::vector<double> tab;            // wx, wxp, grid, pref, brd (re, im)
+    std::vector<double> tab;            // wx, wxp, grid, pref, brd (re, im), w2
     FfLayout LR, LA;
     fk_options oA;                      // options of the I_epsA contraction
     size_t off_tab, off_S, off_VR, off_VA, off_part;
 };
 
-// The evaluation axis as the Fortran computes it, and the bands / frequencies
-// of I_epsA the SX and delta terms read.
+// The evaluation axes as the Fortran (or the C++ port) computes them, and the
+// bands / frequencies of I_epsA the SX and delta terms read.
 void full_axes(const fk_ff_full_problem *p, FullPlan &P)
 {
 #pragma clang fp contract(off)
     const int nb = p->number_bands, ne = p->nfreqeval, nF = p->nfreq;
+    const bool port = p->semantics != FK_FF_FORTRAN;
     P.nb = nb; P.nF = nF; P.ne = ne;
-    P.tab.assign((size_t)2 * nb * ne + 4 * (size_t)nF, 0.0);
+    P.tab.assign((size_t)2 * nb * ne + 4 * (size_t)nF + ne, 0.0);
     double *wx = P.tab.data(), *wxp = wx + (size_t)nb * ne;
     int b_lo = nb, b_hi = -1, f_lo = nF, f_hi = -1;
     for (int n1 = 0; n1 < nb; ++n1) {
         const bool occb = n1 < p->nvband;
         for (int iw = 0; iw < ne; ++iw) {
-            const double w = p->freqevalmin - p->ekq[n1] + (double)iw * p->freqevalstep;
-            wx[(size_t)n1 * ne + iw] = w;
-            wxp[(size_t)n1 * ne + iw] = -p->freqevalmin - p->ekq[n1] + (double)iw * p->freqevalstep;
+            double w;
             int f = -1;
-            if (w >= 0.0) f = find_interval(p->dFreqGrid, nF, w);          // delta terms
-            else if (occb) f = find_interval(p->dFreqGrid, nF, -w);        // SX and delta terms
+            if (!port) {
+                w = p->freqevalmin - p->ekq[n1] + (double)iw * p->freqevalstep;
+                wxp[(size_t)n1 * ne + iw] = -p->freqevalmin - p->ekq[n1] + (double)iw * p->freqevalstep;
+                if (w >= 0.0) f = find_interval(p->dFreqGrid, nF, w);          // delta terms
+                else if (occb) f = find_interval(p->dFreqGrid, nF, -w);        // SX and delta terms
+            } else {
+                w = p->freqevalmin - p->ekq[n1] + p->freqevalstep;                // :140, :224
+                wxp[(size_t)n1 * ne + iw] = p->freqevalmin - p->ekq[n1] + (double)(iw - 1) * p->freqevalstep;
+                if (w > 0.0) {
+                    if (!occb) f = find_interval_strict(p->dFreqGrid, nF, w);    // delta terms only
+                } else if (occb) {
+                    f = find_interval_strict(p->dFreqGrid, nF, -w);
+                    if (f == 0) f = nF - 2;
+                }
+            }
+            wx[(size_t)n1 * ne + iw] = w;
             if (f >= 0) {
                 b_lo = std::min(b_lo, n1); b_hi = std::max(b_hi, n1);
                 f_lo = std::min(f_lo, f); f_hi = std::max(f_hi, f + 1);
             }
         }
+    }
+    double *w2 = wxp + (size_t)nb * ne + 4 * (size_t)nF;
+    if (port) {
+        const int fmin = (int)p->freqevalmin, fstep = (int)p->freqevalstep;   // calculate_schDt_lin3, :10-12
+        for (int iw = 0; iw < ne; ++iw)
+            w2[iw] = (double)fmin - p->ekq[nb - 1] + (double)((iw - 1) * fstep);
     }
     double *t = wxp + (size_t)nb * ne;
     for (int f = 0; f < nF; ++f) {
@@ -346,11 +469,19 @@ int full_launch(const fk_ff_full_problem *p, const fk_options *o, const FullPlan
     a.wx = tab; a.wxp = tab + (size_t)P.nb * P.ne;
     a.grid = a.wxp + (size_t)P.nb * P.ne; a.pref = a.grid + P.nF;
     a.brd = reinterpret_cast<const double2 *>(a.pref + P.nF);
+    a.w2 = a.pref + 3 * (size_t)P.nF;
     a.part = part;
     a.nb = P.nb; a.nv = p->nvband; a.nF = P.nF; a.ne = P.ne; a.nchunks = P.nchunks;
     a.bA0 = P.bA0; a.nbA = P.nbA; a.fA0 = P.fA0; a.nFA = P.nFA;
     a.occ = p->occ; a.pref_zb = p->pref_zb;
-    hipLaunchKernelGGL(ff_epilogue_kernel, dim3(P.nchunks, P.ne), dim3(kEpiThreads), 0, s, a);
+    const dim3 grid(P.nchunks, P.ne);
+    switch (p->semantics) {
+    case FK_FF_CPP_PORT: hipLaunchKernelGGL(ff_epilogue_kernel<FK_FF_CPP_PORT>, grid, dim3(kEpiThreads), 0, s, a); break;
+    case FK_FF_CPP_PORT_ZERO_SCHDT:
+        hipLaunchKernelGGL(ff_epilogue_kernel<FK_FF_CPP_PORT_ZERO_SCHDT>, grid, dim3(kEpiThreads), 0, s, a);
+        break;
+    default: hipLaunchKernelGGL(ff_epilogue_kernel<FK_FF_FORTRAN>, grid, dim3(kEpiThreads), 0, s, a); break;
+    }
     FK_HIP(hipGetLastError());
     hipLaunchKernelGGL(ff_epilogue_sum_kernel, dim3(div_up(P.ne * kTerms, 64)), dim3(64), 0, s, part, P.nchunks,
                        P.ne, io.asx, io.ach, io.cor, io.corb, io.a2);

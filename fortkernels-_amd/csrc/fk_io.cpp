@@ -103,8 +103,8 @@ struct fk_instance {
     void *map = nullptr;
     size_t bytes = 0;
     int32_t kind = 0, complex_bytes = 0;
-    fk_gpp_problem gpp;
-    fk_ff_full_problem ff;
+    fk_gpp_problem gpp{};
+    fk_ff_full_problem ff{};     // semantics 0: an instance holds inputs, not a choice of evaluation
 };
 
 extern "C" {

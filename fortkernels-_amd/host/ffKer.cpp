@@ -16,17 +16,21 @@
 // precision.  A wrong argument count, or ngpown > ncouls, prints the
 // reference's message and exits with status 0 (:305-325).
 //
-// The values follow the Fortran program.  The C++ program departs from it in
-// asxDtemp and achDtemp_cor (its evaluation axis, its aliased sigma-1 output
-// and its never-computed schDt_matrix), so those two printed numbers differ
-// from the C++ program's.  achsDtemp: the C++ program reads frequency index 1
-// (:122), the Fortran the first frequency (:263); the default follows the
-// program this replaces; --achs-freq 0 gives the Fortran.
+// The values follow the Fortran program by default.  The C++ program departs
+// from it in asxDtemp and achDtemp_cor (its evaluation axis, its aliased
+// sigma-1 output and its never-computed schDt_matrix); --semantics 2 prints
+// the C++ program's values instead.  achsDtemp: the C++ program reads
+// frequency index 1 (:122), the Fortran the first frequency (:263); the
+// default follows the program this replaces; --achs-freq 0 gives the
+// Fortran.
 //
 // Options:
 //   --random SEED    seeded inputs generated on the device: aqs*, I_epsR,
 //                    I_epsA ~ U(-0.5,0.5) + iU(-0.5,0.5), vcoul U(0.1,10)
 //   --achs-freq F    frequency index of achsDtemp (default 1)
+//   --semantics N    0: the Fortran program (default); 1: the C++ port's
+//                    arithmetic on the contracted schDt; 2: the C++ program
+//                    as it runs, schDt_matrix zero (fk_ff_semantics)
 //   --mixed          both contractions in split BF16 (FP64 evaluation)
 //   --contraction    only the contraction schDt (ff_kernel), as before the
 //                    evaluation existed; with --both also for I_epsA
@@ -53,8 +57,8 @@ namespace {
 void usage_params()
 {
     std::cout << "./a.out number_bands nvband ncouls ngpown nFreq nfreqeval "
-                 "[--random SEED] [--achs-freq F] [--mixed] [--contraction [--both]] [--repeat K] [--device D] "
-                 "[--dump FILE]\n./a.out --input FILE [--mixed] [--repeat K] [--device D]"
+                 "[--random SEED] [--achs-freq F] [--semantics N] [--mixed] [--contraction [--both]] [--repeat K] "
+                 "[--device D] [--dump FILE]\n./a.out --input FILE [--semantics N] [--mixed] [--repeat K] [--device D]"
               << std::endl;
 }
 
@@ -77,7 +81,7 @@ struct Args {
     bool random = false, both = false, mixed = false, contraction = false;
     std::string input, dump;
     unsigned long long seed = 0;
-    int achs_freq = 1, repeat = 1, device = -1;
+    int achs_freq = 1, repeat = 1, device = -1, semantics = 0;
 };
 
 bool parse(int argc, char **argv, Args &a)
@@ -90,6 +94,7 @@ bool parse(int argc, char **argv, Args &a)
         const char *v = nullptr;
         if (s == "--random") { if (!(v = next())) return false; a.random = true; a.seed = std::strtoull(v, nullptr, 10); }
         else if (s == "--achs-freq") { if (!(v = next())) return false; a.achs_freq = std::atoi(v); }
+        else if (s == "--semantics") { if (!(v = next())) return false; a.semantics = std::atoi(v); }
         else if (s == "--both") a.both = true;
         else if (s == "--mixed") a.mixed = true;
         else if (s == "--contraction") a.contraction = true;
@@ -201,7 +206,9 @@ int run_input(const Args &a)
     opt.memory = FK_MEM_HOST;
     opt.precision = a.mixed ? FK_PREC_MIXED : FK_PREC_FP64;
     opt.device = a.device;
-    run_full_and_print(*p, opt, a);
+    fk_ff_full_problem q = *p;
+    q.semantics = a.semantics;
+    run_full_and_print(q, opt, a);
     fk_instance_close(inst);
     return 0;
 }
@@ -310,6 +317,7 @@ int main(int argc, char **argv)
         p.ekq = ekq.data(); p.dFreqGrid = grid.data(); p.pref = pref.data(); p.dFreqBrd = brd.data();
         p.pref_zb = 0.5 / 3.14; p.freqevalmin = 0.0; p.freqevalstep = 0.5; p.occ = 1.0;
         p.achs_freq = a.achs_freq;
+        p.semantics = a.semantics;
         if (!a.dump.empty()) dump_instance(a.dump, p, a.random);
         run_full_and_print(p, opt, a);
     } else {

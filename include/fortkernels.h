@@ -281,12 +281,38 @@ void ff_batch_destroy(fk_ff_batch *batch);
  *   achDtemp_cor[iw]  sigma 4 + delta terms
  * summed over bands n1 (occupied: n1 < nvband).  Interval search, its
  * fall-back to the last interval, the 1e-4..1e4 clamps of the log ratios
- * and the band/frequency ranges of each term follow the Fortran.  The C++
- * port FFKer_cori/ffKernel.cpp:10-300 departs from the Fortran (evaluation
- * axis, aliased outputs, an uncomputed schDt_matrix) and is not reproduced.
+ * and the band/frequency ranges of each term follow the Fortran.
+ *
+ * semantics = FK_FF_CPP_PORT reproduces instead the C++ port
+ * FFKer_cori/ffKernel.cpp:10-300 as its main calls it (:485-500):
+ *   asxDtemp[iw]      SX from VR only (:148), at wx' = freqevalmin - ekq[n1]
+ *                     + freqevalstep for every iw (:140), with strict
+ *                     interval tests and the nfreq-2 fall-back of
+ *                     compute_fact (:30-78); plus sigma 1 over every
+ *                     frequency at freqevalmin - ekq[n1] + (iw-1)
+ *                     freqevalstep (:199-206; main passes asxDtemp as
+ *                     achDtemp, :495)
+ *   achDtemp[iw], achDtemp_corb[iw]   zero
+ *   achDtemp_cor[iw]  the delta terms at wx' (:212-300; sigma 4 is reset
+ *                     before it is read, :222-223)
+ *   ach2Dtemp[iw]     nb times the sigma-2 value of the last band and
+ *                     frequency (sch2Di is reset per term, :191), its ratio
+ *                     taken without abs on int(freqevalmin) - ekq +
+ *                     (iw-1) int(freqevalstep) (calculate_schDt_lin3, :10-27)
+ * FK_FF_CPP_PORT_ZERO_SCHDT evaluates the same with schDt_matrix = 0, as the
+ * C++ program leaves it (:421-422), so that sigma 1 and ach2Dtemp vanish;
+ * the schDt output still holds the contraction.  achs_freq is independent
+ * (1 as the C++ program).
+ *
  * I_epsA is contracted only over the bands and frequencies the SX and delta
  * terms read.
  * ------------------------------------------------------------------------ */
+typedef enum {
+    FK_FF_FORTRAN = 0,               /* ffkernel.deslippe.best.f90 (default) */
+    FK_FF_CPP_PORT = 1,              /* FFKer_cori/ffKernel.cpp arithmetic */
+    FK_FF_CPP_PORT_ZERO_SCHDT = 2    /* the same, schDt_matrix = 0 */
+} fk_ff_semantics;
+
 typedef struct {
     int32_t number_bands;        /* nb */
     int32_t nvband;              /* occupied bands, 0..nb */
@@ -310,6 +336,7 @@ typedef struct {
     double freqevalstep;         /* 0.5 */
     double occ;                  /* 1 */
     int32_t achs_freq;           /* 0 as the Fortran */
+    int32_t semantics;           /* fk_ff_semantics; 0 = the Fortran */
 } fk_ff_full_problem;
 
 typedef struct {

@@ -188,6 +188,31 @@ def test_ff_full_workspace_and_checks(fk, fko):
     assert e.value.code == fk.FK_ERR_ARG and "increasing" in fk.last_error()
 
 
+def test_ff_full_semantics_checks(fk, fko):
+    """The C++ port's semantics read I_epsA for other bands (wx' without iw,
+    SX from I_epsR only), take freqevalmin / freqevalstep as int, and an
+    unknown semantics value is rejected."""
+    nb, C, G, nf, ne = 64, 512, 256, 16, 10
+    ekq, grid, _, _ = fko.ff_program_axes(nb, nf)
+    ws = {s: fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=s) for s in (0, 1, 2)}
+    assert all(v > 0 for v in ws.values()) and ws[1] == ws[2]
+    # every band unoccupied and wx' <= 0: the port reads no I_epsA at all
+    assert (fk.ff_full_workspace_size(nb, 0, C, G, nf, ne, ekq + 100.0, grid, semantics=1)
+            == fk.ff_full_workspace_size(nb, 0, C, G, nf, ne, ekq + 100.0, grid, semantics=0))
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=3) == 0
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=-1) == 0
+    big = dict(freqevalmin=3e9)
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=0, **big) > 0
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=1, **big) == 0
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, ne, ekq, grid, semantics=1, freqevalstep=float("nan")) == 0
+    # (nfreqeval-2) * int(freqevalstep) overflows int32
+    assert fk.ff_full_workspace_size(nb, 8, C, G, nf, 40000, ekq, grid, semantics=2, freqevalstep=1e5) == 0
+    p = fko.ff_full_random(6, 2, 16, 4, 3, 4, seed=1)
+    with pytest.raises(fk.FkError) as e:
+        fk.ff_full_problem(p, semantics=7)
+    assert e.value.code == fk.FK_ERR_ARG and "semantics" in fk.last_error()
+
+
 def test_ff_batch_rejects_host_memory_and_bad_config(fk, fko):
     """Argument checks of ff_batch_create run before any device work."""
     import ctypes

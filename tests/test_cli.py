@@ -121,6 +121,40 @@ def test_ffker_matches_reference_program(fko):
 
 
 @pytest.mark.gpu
+def test_ffker_cpp_semantics_prints_the_cpp_program(fko):
+    """--semantics 2 prints what the C++ ffKernel program prints
+    (ff_main.npz: its asxDtemp[0] and achDtemp_cor[0] at %f, achsDtemp at
+    full precision), and every output at full precision as the oracle's
+    variant 1 with schDt_matrix zero."""
+    d = np.load(os.path.join(GOLDEN, "ff_main.npz"))
+    for (nb, nv, C, G, nf, nfe), achs, asx, cor in zip(d["shapes"], d["achsDtemp"], d["asxDtemp"],
+                                                        d["achDtemp_cor"]):
+        nb, nv, C, G, nf, nfe = (int(x) for x in (nb, nv, C, G, nf, nfe))
+        print(f"ffKer {nb} {nv} {C} {G} {nf} {nfe} --semantics 2", flush=True)
+        r = _run("ffKer", nb, nv, C, G, nf, nfe, "--semantics", 2)
+        assert r.returncode == 0, r.stderr
+        printed = {}
+        for name in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+            m = re.search(rf"^{name} = \( (-?\d+\.\d{{6}}), (-?\d+\.\d{{6}})\) $", r.stdout, re.M)
+            printed[name] = complex(float(m.group(1)), float(m.group(2)))
+        for name, ref in (("achsDtemp", achs), ("asxDtemp", asx), ("achDtemp_cor", cor)):
+            assert abs(printed[name] - ref) <= 1e-6 * max(1.0, abs(ref)), (name, printed[name], ref)
+        ekq, grid, pref, brd = fko.ff_program_axes(nb, nf)
+        z = np.full((nb, C), 0.5 + 0.5j)
+        inv = np.arange(G, dtype=np.int32)
+        ind = np.arange(C + 1, dtype=np.int32)
+        ind[C] = C - 1
+        p = fko.FfFullProblem(nb, nv, C, G, nf, nfe, z, z.copy(), np.full((nf, G, C), 0.5 + 0.5j),
+                              np.full((nf, G, C), 0.5 - 0.5j), np.ones(C), inv, ind, ekq, grid, pref, brd,
+                              achs_freq=1)
+        want = fko.ff_full(p, variant=1)
+        for name in fko.FF_OUTPUTS:
+            got = np.array(_complex_lines(r.stdout, name))
+            scale = max(np.max(np.abs(want[name])), 1e-300)
+            assert np.max(np.abs(got - want[name])) <= 1e-12 * scale, name
+
+
+@pytest.mark.gpu
 def test_ffker_random_matches_oracle(fko):
     """--random fills aqs*, I_epsR, I_epsA, vcoul on the device with the seeded
     generator the oracle shares; every printed output equals the oracle's."""

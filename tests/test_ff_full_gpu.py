@@ -231,3 +231,76 @@ def test_full_size_random_mixed_vs_fp64(fk, fko):
     _check(res["mixed"], res["fp64"], MIXED_RTOL, "full size mixed vs fp64")
     del IR, IA
     torch.cuda.empty_cache()
+
+
+# ---------------------------------------------------------------------------
+# semantics FK_FF_CPP_PORT / FK_FF_CPP_PORT_ZERO_SCHDT: the C++ port
+# FFKer_cori/ffKernel.cpp as its main calls it
+# ---------------------------------------------------------------------------
+def _fixture(fko, name):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", f"ff_full_{name}.npz"))
+    nb, nv, C, G, nF, ne = (int(x) for x in d["sizes"])
+    pz, fmin, fstep, occ = (float(x) for x in d["scalars"])
+    p = fko.FfFullProblem(nb, nv, C, G, nF, ne, d["aqsm"], d["aqsn"], d["I_epsR"], d["I_epsA"], d["vcoul"],
+                          d["inv_igp_index"], d["indinv"], d["ekq"], d["dFreqGrid"], d["pref"], d["dFreqBrd"],
+                          pz, fmin, fstep, occ, achs_freq=1)
+    return p, d
+
+
+def test_cpp_port_matches_reference_functions(fk, fko):
+    """ff_full_program_axes.npz holds what the C++ port's own kernel functions
+    (compiled from FFKer_cori/ffKernel.cpp, one thread) return for these
+    inputs with schDt_matrix = the contraction of I_epsR, which is what the
+    device evaluates with under FK_FF_CPP_PORT."""
+    p, d = _fixture(fko, "program_axes")
+    print("ff_full cpp port vs reference functions, program axes", flush=True)
+    got = _np(fk.ff_full_problem(_to_dev(p), semantics=fk.FK_FF_CPP_PORT))
+    for k in OUTS:
+        ref = d[k]
+        if np.max(np.abs(ref)) == 0:
+            assert np.max(np.abs(got[k])) == 0, k
+        else:
+            assert _rel(got[k], ref) <= RTOL, (k, _rel(got[k], ref))
+    assert abs(got["achsDtemp"] - complex(d["achsDtemp"])) <= RTOL * abs(complex(d["achsDtemp"]))
+
+
+@pytest.mark.parametrize("case", [("seeded", 0), ("seeded", 1), ("variety", 3), ("variety", 4)])
+def test_cpp_port_vs_oracle(fk, fko, case):
+    """The port's arithmetic on inputs that exercise it: a non-uniform grid,
+    band energies that put wx' on both sides of zero, freqevalmin and
+    freqevalstep that the port truncates to int (0.25, 1.5).  Against the
+    oracle's variant 1 (pinned to the compiled port) with the oracle's own
+    contraction as schDt_matrix (FK_FF_CPP_PORT) and with zeros
+    (FK_FF_CPP_PORT_ZERO_SCHDT, the C++ program as it runs)."""
+    kind, seed = case
+    if kind == "seeded":
+        p, _ = _fixture(fko, "seeded")
+        if seed:
+            p.freqevalmin, p.freqevalstep = -1.75, 2.5
+    else:
+        p = _variety(fko, seed, *CASES[seed - 1][1:7], fmin=0.6, step=1.3, random_axes=True)
+        p.achs_freq = 1
+    print(f"ff_full cpp port case {case}", flush=True)
+    S = fko.ff_full(p)["schDt"]
+    for sem, schDt_in in ((fk.FK_FF_CPP_PORT, S), (fk.FK_FF_CPP_PORT_ZERO_SCHDT, None)):
+        ref = fko.ff_full(p, variant=1, schDt_in=schDt_in)
+        for memory in ("device", "host"):
+            got = _np(fk.ff_full_problem(_to_dev(p) if memory == "device" else p, semantics=sem))
+            for k in OUTS:
+                if np.max(np.abs(ref[k])) == 0:
+                    assert np.max(np.abs(got[k])) == 0, (sem, memory, k)
+                else:
+                    assert _rel(got[k], ref[k]) <= RTOL, (sem, memory, k, _rel(got[k], ref[k]))
+            assert _rel(got["schDt"], S) <= RTOL
+            assert abs(got["achsDtemp"] - ref["achsDtemp"]) <= RTOL * abs(ref["achsDtemp"])
+
+
+def test_cpp_port_mixed(fk, fko):
+    p, _ = _fixture(fko, "seeded")
+    print("ff_full cpp port mixed", flush=True)
+    ref = fko.ff_full(p, variant=1, schDt_in=fko.ff_full(p)["schDt"])
+    got = _np(fk.ff_full_problem(_to_dev(p), semantics=fk.FK_FF_CPP_PORT, precision="mixed"))
+    errs = {k: _rel(got[k], ref[k]) for k in OUTS if np.max(np.abs(ref[k])) > 0}
+    print("  " + " ".join(f"{k} {v:.1e}" for k, v in errs.items()), flush=True)
+    assert max(errs.values()) <= MIXED_RTOL
