@@ -180,6 +180,13 @@ int gpp_kernel_ex(const fk_gpp_problem *problem, const fk_options *options, cons
  * reduction.  Inputs are device memory (FK_MEM_DEVICE) and are read again
  * on every run, so new values in the same arrays need no new batch.
  * Problems: nfreq <= 16; the arrays must outlive the batch.
+ *
+ * options->precision = FK_PREC_MIXED takes complex64 problems and runs the
+ * persistent tile kernel with FP32 MFMA (configs as for FP64; error ~1e-6
+ * against FP64 on the same values).  A form on the int8-split contraction
+ * of gpp_kernel's mixed path (a graph of per-problem launches on parallel
+ * branches) exists in the source but is switched off.
+ *
  * gpp_batch_run launches the graph on `stream` and writes achtemp[i],
  * asxtemp[i] (device pointers, [nfreq_i] each); elapsed_ms (optional)
  * receives the graph's duration and synchronises.  Calls that share one
@@ -192,8 +199,12 @@ int gpp_batch_create(int32_t n, const fk_gpp_problem *const *problems, const fk_
                      fk_gpp_batch **batch);
 int gpp_batch_run(fk_gpp_batch *batch, fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, void *stream,
                   float *elapsed_ms);
-/* Tile configuration the batch uses (1-based, as fk_options.config). */
+/* Tile configuration the batch uses (1-based, as fk_options.config; 0 for
+ * the int8-split graph). */
 int gpp_batch_config(const fk_gpp_batch *batch);
+/* Branches of the int8-split graph (0 for the persistent tile kernel, which
+ * every batch runs while that graph is switched off). */
+int gpp_batch_branches(const fk_gpp_batch *batch);
 size_t gpp_batch_workspace_bytes(const fk_gpp_batch *batch);
 void gpp_batch_destroy(fk_gpp_batch *batch);
 int gpp_kernel_batched(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,

@@ -101,21 +101,94 @@ def test_batch_graph_replays_read_current_inputs(fk, fko):
     b.close()
 
 
-def test_batch_mixed_precision(fk, fko):
-    """Mixed batches use the FP32 MFMA contraction; error against the FP64
-    oracle on the same complex64 values is reported and bounded."""
+MIXED_RTOL = 1e-8      # the int8-split contraction, as gpp_kernel's mixed path
+INT8_BATCH_OFF = ("int8-split batch graph switched off: its first MI355X run ended in a host segmentation "
+                  "fault inside gpp_batch_create (multi-stream graph capture); cause not found")
+
+
+def _mixed_single(fk, p):
+    return fk.gpp_kernel(p["nb"], p["nv"], p["ncouls"], p["ngpown"], p["wx"], p["aqsm"], p["aqsn"], p["I_eps"],
+                         p["wtilde"], p["vcoul"], p["inv_igp_index"], p["indinv"], precision="mixed")
+
+
+@pytest.mark.skip(reason=INT8_BATCH_OFF)
+def test_batch_mixed_int8_split(fk, fko):
+    """Mixed batches with config 0 run the int8-split contraction: each
+    problem's outputs equal gpp_kernel's mixed result bit for bit (the same
+    kernels on the same values), and are within 1e-8 of the FP64 oracle on
+    the same complex64 values."""
     torch = _torch()
     probs, oracle = _problems(fko, SHAPES, cdt=torch.complex64, round64=True)
     b = fk.GppBatch(probs, precision="mixed", config=0)
     ach, asx, _ = b.run()
     torch.cuda.synchronize()
+    print(f"int8-split batch: config {b.config}, {b.branches} branches, workspace {b.workspace_bytes} B", flush=True)
+    assert b.config == 0 and b.branches in (1, 2, 4, 8)
+    worst = 0.0
+    for k, p in enumerate(oracle):
+        a1, s1, _ = _mixed_single(fk, probs[k])
+        torch.cuda.synchronize()
+        assert torch.equal(ach[k], a1) and torch.equal(asx[k], s1), f"problem {k}"
+        ra, rs, _ = fko.gpp(p)
+        worst = max(worst, _rel(ach[k].cpu().numpy(), ra))
+        if p.nv:
+            worst = max(worst, _rel(asx[k].cpu().numpy(), rs))
+        else:
+            assert np.all(asx[k].cpu().numpy() == 0)
+    print(f"int8-split batch worst relative error {worst:.2e}", flush=True)
+    assert worst <= MIXED_RTOL
+    b.close()
+
+
+@pytest.mark.skip(reason=INT8_BATCH_OFF)
+def test_batch_mixed_int8_split_replays_and_no_band_problem(fk, fko):
+    """Replays of the int8-split graph see the inputs' current values, and a
+    problem without bands gives zeros."""
+    torch = _torch()
+    probs, _ = _problems(fko, SHAPES[:3], cdt=torch.complex64, round64=True)
+    probs.append(dict(nb=0, nv=0, ncouls=128, ngpown=6, wx=np.array([3.0, 4.0, 5.0, 6.0]),
+                      aqsm=torch.zeros(1, dtype=torch.complex64, device="cuda"),
+                      aqsn=torch.zeros(1, dtype=torch.complex64, device="cuda"),
+                      I_eps=torch.zeros(6 * 128, dtype=torch.complex64, device="cuda"),
+                      wtilde=torch.ones(6 * 128, dtype=torch.complex64, device="cuda"),
+                      vcoul=torch.ones(128, dtype=torch.float64, device="cuda"),
+                      inv_igp_index=np.arange(6, dtype=np.int32) * 20, indinv=np.arange(129, dtype=np.int32)))
+    b = fk.GppBatch(probs, precision="mixed", config=0)
+    base_a, base_s, _ = b.run()
+    base_a = [t.clone() for t in base_a]
+    base_s = [t.clone() for t in base_s]
+    assert torch.count_nonzero(base_a[3]) == 0 and torch.count_nonzero(base_s[3]) == 0
+    for rep in range(3):
+        for p in probs[:3]:
+            p["vcoul"].mul_(2.0)
+        a, s, ms = b.run(time_it=True)
+        torch.cuda.synchronize()
+        assert ms is not None and ms > 0
+        for k in range(3):
+            assert torch.equal(a[k], base_a[k] * 2.0 ** (rep + 1)), f"replay {rep} problem {k}"
+            assert torch.equal(s[k], base_s[k] * 2.0 ** (rep + 1)), f"replay {rep} problem {k}"
+        _ = torch.ones(1 << 20, device="cuda").sum()   # eager work between replays
+    b.close()
+
+
+@pytest.mark.parametrize("config", [0, 1, 4])
+def test_batch_mixed_fp32_tiles(fk, fko, config):
+    """Mixed batches use the FP32 MFMA tile kernel (config 0: the autotuned
+    configuration); its error against the FP64 oracle on the same complex64
+    values is reported and bounded."""
+    torch = _torch()
+    probs, oracle = _problems(fko, SHAPES, cdt=torch.complex64, round64=True)
+    b = fk.GppBatch(probs, precision="mixed", config=config)
+    ach, asx, _ = b.run()
+    torch.cuda.synchronize()
+    assert (b.config == config or (config == 0 and 1 <= b.config <= 4)) and b.branches == 0
     worst = 0.0
     for k, p in enumerate(oracle):
         ra, rs, _ = fko.gpp(p)
         worst = max(worst, _rel(ach[k].cpu().numpy(), ra))
         if p.nv:
             worst = max(worst, _rel(asx[k].cpu().numpy(), rs))
-    print(f"mixed batch worst relative error {worst:.2e} (config {b.config})", flush=True)
+    print(f"FP32 tile batch worst relative error {worst:.2e} (config {b.config})", flush=True)
     assert worst < 1e-5
     b.close()
 
@@ -144,3 +217,27 @@ def test_kernel_batched_one_call(fk, fko):
     for k, p in enumerate(oracle):
         ra, rs, _ = fko.gpp(p)
         assert _rel(ach[k].cpu().numpy(), ra) <= RTOL and _rel(asx[k].cpu().numpy(), rs) <= RTOL
+
+
+@pytest.mark.skip(reason=INT8_BATCH_OFF)
+def test_kernel_batched_one_call_mixed(fk, fko):
+    """gpp_kernel_batched with FK_PREC_MIXED and config 0: the int8-split
+    graph on four branches, equal to gpp_kernel's mixed result."""
+    import ctypes
+    torch = _torch()
+    probs, _ = _problems(fko, SHAPES[:3], cdt=torch.complex64, round64=True)
+    b = fk.GppBatch(probs, precision="mixed", config=0)     # keeps the structs alive
+    arr = (ctypes.POINTER(fk.GppProblem) * 3)(*[ctypes.pointer(s) for s in b._structs])
+    ach = [torch.empty(len(p["wx"]), dtype=torch.complex128, device="cuda") for p in probs]
+    asx = [torch.empty(len(p["wx"]), dtype=torch.complex128, device="cuda") for p in probs]
+    pa = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in ach])
+    px = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in asx])
+    opt = fk.default_options()
+    opt.memory = fk.FK_MEM_DEVICE
+    opt.precision = fk.FK_PREC_MIXED
+    assert fk.lib.gpp_kernel_batched(3, arr, ctypes.byref(opt), pa, px) == 0, fk.last_error()
+    for k, p in enumerate(probs):
+        a1, s1, _ = _mixed_single(fk, p)
+        torch.cuda.synchronize()
+        assert torch.equal(ach[k], a1) and torch.equal(asx[k], s1), f"problem {k}"
+    b.close()
