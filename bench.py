@@ -318,7 +318,7 @@ def batched_shapes(count=64, seed=3):
     return out
 
 
-def run_batched(fk, torch, steps):
+def run_batched(fk, torch, steps, singles=False):
     """64 problems as one batch, FP64 then mixed (complex64 storage, FP32
     MFMA tile kernel); inputs resident in HBM."""
     shapes = batched_shapes()
@@ -371,6 +371,18 @@ def run_batched(fk, torch, steps):
                 del up
             result[prec]["error_vs_fp64_same_values"] = {"max": worst, "problems": len(range(0, len(probs), 8)),
                                                          "norm": "max|diff|/max|ref| per output"}
+        if prec == "fp64" and singles:
+            # each problem alone through gpp_kernel, best tile configuration
+            # per problem, fused-kernel time (kernel_ms)
+            tot = 0.0
+            for p in probs:
+                best = min(min(fk.gpp_kernel(p["nb"], p["nv"], p["ncouls"], p["ngpown"], p["wx"], p["aqsm"],
+                                             p["aqsn"], p["I_eps"], p["wtilde"], p["vcoul"], p["inv_igp_index"],
+                                             p["indinv"], config=c, time_kernel=True)[2] for _ in range(3))
+                           for c in (1, 2, 3, 4))
+                tot += best
+            result[prec]["sum_of_single_problem_kernel_ms"] = tot
+            log(f"batched {prec}: the 64 problems one by one, best configuration each: {tot:.2f} ms of kernels")
         if prec == "fp64":
             fl = sum(gpp_alg_flops(nb, nv, C, G, nw) for nb, nv, C, G, nw in shapes)
             result[prec]["achieved_tflops"] = fl / (ms * 1e-3) / 1e12
@@ -432,6 +444,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ff", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-singles", action="store_true",
+                    help="also time each batched FP64 problem alone (gpp_kernel, best configuration)")
     ap.add_argument("--cpu-bands", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -495,7 +509,7 @@ def main():
     if not args.no_ff:
         line["ff"] = run_ff(fk, torch, max(3, args.steps // 4), 1)
     if not args.no_batched:
-        line["batched"] = run_batched(fk, torch, 3)
+        line["batched"] = run_batched(fk, torch, 3, singles=args.batch_singles)
     if args.calibrate_copy:
         n = args.calibrate_copy << 20
         src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")

@@ -58,6 +58,45 @@ struct GppArgs {
     PoleParams pp;
 };
 
+// Loads and stores through a pointer into global memory, as global (not
+// flat) instructions.  The single-problem kernel's pointers are kernel
+// arguments and known to be global; the batch kernel reads them from its
+// problem table, which leaves them generic, and flat instructions take
+// 64-bit addresses per lane and also count against the LDS counter.  The
+// element is moved as a plain vector: the HIP vector types' copy operations
+// would turn the access back into a generic one.
+template <class T> struct RawOf { typedef T type; };
+template <> struct RawOf<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct RawOf<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// GLOBAL = false: a plain access (the single-problem kernel keeps the code
+// the compiler makes for its kernel-argument pointers).
+template <bool GLOBAL, class T>
+__device__ __forceinline__ T gld(const T *p, size_t i)
+{
+    if constexpr (!GLOBAL) {
+        return p[i];
+    } else {
+        typedef typename RawOf<T>::type R;
+        const R v = *(const __attribute__((address_space(1))) R *)(p + i);
+        if constexpr (__is_same(R, T)) {
+            return v;
+        } else {
+            T e;
+            e.x = v.x;
+            e.y = v.y;
+            return e;
+        }
+    }
+}
+
+template <bool GLOBAL>
+__device__ __forceinline__ void gst(double *p, size_t i, double v)
+{
+    if constexpr (!GLOBAL) p[i] = v;
+    else *(__attribute__((address_space(1))) double *)(p + i) = v;
+}
+
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -187,7 +226,9 @@ struct TileLds {
 // ACHS: also the static-CH remainder of the tile, -sum B I_eps over its kept
 // (my_igp, ig) (gppkernel.deslippe.09.30.2014.f90:192-230; B carries the
 // 0.5 vcoul of the coefficients), into achs_part[tile].
-template <class E, int MB, int NB, int P, int WGM, int WGN, int NWC, bool ACHS>
+// TABLE: the arguments come from the batch kernel's problem table (global
+// instructions through gld / gst).
+template <class E, int MB, int NB, int P, int WGM, int WGN, int NWC, bool ACHS, bool TABLE = false>
 __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, TileLds<E, MB, NB, WGM * WGN, NWC> &sh)
 {
     using A4 = typename Acc<E>::type;
@@ -239,13 +280,13 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
         s = min(s, a.nks - 1);
         const E *qa = pa + (size_t)(4 * s) * a.ldc;
 #pragma unroll
-        for (int i = 0; i < MB; ++i) f.a[i] = qa[i * 16];
+        for (int i = 0; i < MB; ++i) f.a[i] = gld<TABLE>(qa, i * 16);
         const int kr = 4 * s + kl;
         int band = kr < a.nv ? kr : kr - a.nvpad + a.nv;
         band = min(max(band, 0), a.nb - 1);
         const E *qb = a.aqsn + (size_t)band * a.C;
 #pragma unroll
-        for (int j = 0; j < NB; ++j) f.b[j] = qb[colj[j]];
+        for (int j = 0; j < NB; ++j) f.b[j] = gld<TABLE>(qb, colj[j]);
     };
     auto mma = [&](const Frags<E, MB, NB> &f) {
 #pragma unroll
@@ -293,7 +334,7 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = rowbase + i * 16 + acc_row<E>(lane, r);
-            const bool ok = row < a.G && a.igp[min(row, a.G - 1)] >= 0;
+            const bool ok = row < a.G && gld<TABLE>(a.igp, min(row, a.G - 1)) >= 0;
             rowok |= (ok ? 1u : 0u) << (i * 4 + r);
         }
 
@@ -313,8 +354,8 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
                 for (int r = 0; r < 4; ++r) {
                     const int row = rowbase + i * 16 + acc_row<E>(lane, r);
                     const size_t e = (size_t)min(row, a.G - 1) * a.C + min(col, nmax);
-                    cwt[r] = a.wtilde[e];
-                    cie[r] = a.I_eps[e];
+                    cwt[r] = gld<TABLE>(a.wtilde, e);
+                    cie[r] = gld<TABLE>(a.I_eps, e);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -347,8 +388,8 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
                 double sum = red[0][tid];
 #pragma unroll
                 for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
-                double *dst = reinterpret_cast<double *>(a.partials + (tile * a.nw + iw) * 2);
-                dst[comp] = sum;   // comp 0,1: achtemp re,im; 2,3: asxtemp re,im
+                // comp 0,1: achtemp re,im; 2,3: asxtemp re,im
+                gst<TABLE>(reinterpret_cast<double *>(a.partials + (tile * a.nw + iw) * 2), comp, sum);
             }
         }
         __syncthreads();
@@ -364,7 +405,7 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
             double sum = red[0][tid];
 #pragma unroll
             for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
-            reinterpret_cast<double *>(a.achs_part + tile)[tid] = sum;
+            gst<TABLE>(reinterpret_cast<double *>(a.achs_part + tile), tid, sum);
         }
         __syncthreads();
     }
@@ -394,20 +435,25 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
 struct BatchTile { int prob, tm, tn; };
 
 template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
-__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_batch_kernel(const GppArgs<E> *probs,
-                                                                         const BatchTile *tiles, int ntiles,
-                                                                         int *counter)
+__global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_batch_kernel(const GppArgs<E> *__restrict__ probs,
+                                                                         const BatchTile *__restrict__ tiles,
+                                                                         int ntiles, int *counter)
 {
     __shared__ TileLds<E, MB, NB, WGM * WGN, NWC> sh;
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = next;
+        // one value for the whole workgroup: read-only, uniform addresses let
+        // the problem's arguments live in scalar registers, as a kernel
+        // argument does in the single-problem kernel
+        const int t = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
-        gpp_tile<E, MB, NB, P, WGM, WGN, NWC, false>(probs[bt.prob], bt.tm, bt.tn, sh);
+        const int prob = __builtin_amdgcn_readfirstlane(bt.prob);
+        const int tm = __builtin_amdgcn_readfirstlane(bt.tm), tn = __builtin_amdgcn_readfirstlane(bt.tn);
+        gpp_tile<E, MB, NB, P, WGM, WGN, NWC, false, true>(probs[prob], tm, tn, sh);
     }
 }
 
@@ -845,7 +891,9 @@ __global__ void batch_scatter_kernel(const double2 *out, const int *nw, double2 
     }
 }
 
-constexpr int kBatchNwc = 3;   // frequencies per epilogue pass in batches
+// frequencies per epilogue pass in batches (3..11 frequencies: 4 takes the
+// mixed batch from 93 to 87 ms, the FP64 one unchanged at 136 ms)
+constexpr int kBatchNwc = 4;
 
 template <class E>
 void launch_batch(int cfg, int grid, hipStream_t s, const GppArgs<E> *probs, const BatchTile *tiles, int ntiles,

@@ -62,7 +62,8 @@ def test_batch_vs_oracle(fk, fko, config):
 
 def test_batch_equals_single_problem_calls(fk, fko):
     """Same tile configuration -> the same tiles and the same fixed-order sums
-    as gpp_kernel, except that batches run 3 frequencies per epilogue pass."""
+    as gpp_kernel, except that batches run 4 frequencies per epilogue pass
+    where gpp_kernel runs up to 3 in one pass."""
     torch = _torch()
     probs, _ = _problems(fko, SHAPES[:3])
     b = fk.GppBatch(probs, config=2)
