@@ -399,6 +399,72 @@ def run_batched(fk, torch, steps, singles=False):
     return result
 
 
+def ff_batched_shapes(count=24, seed=5):
+    """number_bands ~U{32..256}, ncouls ~U{512..3072} rounded to 64, ngpown
+    = ncouls/2, nFreq ~U{8..32}: many small FF contractions."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nb = int(rng.integers(32, 257))
+        C = int(rng.integers(512, 3073)) // 64 * 64
+        out.append((nb, C, C // 2, int(rng.integers(8, 33))))
+    return out
+
+
+def run_ff_batched(fk, torch, steps):
+    """24 uneven FF contractions: one ff_batch graph launch against the same
+    problems through ff_kernel one after another (same stream, inputs
+    resident in HBM), FP64 and split BF16, configuration autotuned."""
+    shapes = ff_batched_shapes()
+    dev = torch.device("cuda:0")
+    flops = float(sum(8.0 * G * nf * nb * C for nb, C, G, nf in shapes))
+    res = {"workload": "24 FF contractions: number_bands U{32..256}, ncouls U{512..3072}/64*64, ngpown ncouls/2, "
+                       "nFreq U{8..32}", "flops": flops}
+    probs = []
+    for k, (nb, C, G, nf) in enumerate(shapes):
+        t = {"aqsm": torch.empty(nb, C, dtype=torch.complex128, device=dev),
+             "aqsn": torch.empty(nb, C, dtype=torch.complex128, device=dev),
+             "I_eps": torch.empty(nf, G, C, dtype=torch.complex128, device=dev),
+             "vcoul": torch.empty(C, dtype=torch.float64, device=dev)}
+        for i, name in enumerate(("aqsm", "aqsn", "I_eps")):
+            fk.fill_complex(t[name], 200 + k, 1 + i, (-0.5, 0.5), (-0.5, 0.5))
+        fk.fill_real(t["vcoul"], 200 + k, 5, 0.1, 10.0)
+        inv, indinv = index_maps(C, G)
+        probs.append(dict(nb=nb, ncouls=C, ngpown=G, nfreq=nf, inv_igp_index=inv, indinv=indinv, **t))
+    torch.cuda.synchronize()
+    for prec in ("fp64", "mixed"):
+        b = fk.FfBatch(probs, precision=prec, config=0)
+        b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+        torch.cuda.synchronize()
+        ms_b = 1e3 * (time.perf_counter() - t0) / steps
+
+        def singles():
+            for p in probs:
+                fk.ff_kernel(p["nb"], p["ncouls"], p["ngpown"], p["nfreq"], p["aqsm"], p["aqsn"], p["I_eps"],
+                             p["vcoul"], p["inv_igp_index"], p["indinv"], config=b.config, precision=prec)
+
+        singles()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            singles()
+        torch.cuda.synchronize()
+        ms_s = 1e3 * (time.perf_counter() - t0) / steps
+        res[prec] = {"ms_per_step": ms_b, "config": b.config, "ms_one_call_per_problem": ms_s,
+                     "speedup_vs_one_call_per_problem": ms_s / ms_b,
+                     "achieved_tflops": flops / (ms_b * 1e-3) / 1e12}
+        log(f"ff batched {prec}: {ms_b:.3f} ms per batch (config {b.config}) against {ms_s:.3f} ms "
+            f"one call per problem")
+        b.close()
+    del probs
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(bands, threads):
     """The reference's own gppKer (oracle/_ref) on `bands` bands of the
     512/2/32768/20 shape, OMP threads = `threads`."""
@@ -510,6 +576,7 @@ def main():
         line["ff"] = run_ff(fk, torch, max(3, args.steps // 4), 1)
     if not args.no_batched:
         line["batched"] = run_batched(fk, torch, 3, singles=args.batch_singles)
+        line["ff_batched"] = run_ff_batched(fk, torch, 5)
     if args.calibrate_copy:
         n = args.calibrate_copy << 20
         src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")

@@ -37,7 +37,8 @@ struct FfArgs {
     int nb, C, G, nfreq, nks;
     int tiles_m, tiles_n, mtiles;   // tiles_n = nfreq * mtiles
     int stagger_lo, stagger_hi, stagger_ticks;
-    int ksplit;             // ff_fewband_kernel: ig ranges per column block
+    int ksplit;             // ig ranges per column block (1: the whole range)
+    int kper;               // ff_fused_kernel with KSPLIT: k-steps per range (a multiple of P)
 };
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
@@ -85,7 +86,10 @@ struct Frags {
 // 16x16x4 FP64 MFMA: lane l supplies A[l%16][l/16] (here aqsn[n1 = row l%16]
 // [ig = l/16]) and B[l/16][l%16] (X[m = col l%16][ig = l/16]); accumulator
 // register r of lane l holds T[4r + l/16][l%16].
-template <int MB, int NB, int P, int WGM, int WGN, int MINW>
+// KSPLIT: the ig range is split into ksplit parts of kper k-steps, one per
+// workgroup (column tile index = column block * ksplit + part), for problems
+// with too few tiles to fill the GPU; each part writes its own partial.
+template <int MB, int NB, int P, int WGM, int WGN, int MINW, bool KSPLIT>
 __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a)
 {
     constexpr int NWAVES = WGM * WGN;
@@ -101,8 +105,11 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
     int tm, tn;
     tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
     const int f = tn / a.mtiles, mt = tn % a.mtiles;
+    const int kpart = KSPLIT ? mt % a.ksplit : 0;
+    const int s_beg = KSPLIT ? kpart * a.kper : 0;
+    const int s_end = KSPLIT ? min(a.nks, s_beg + a.kper) : a.nks;
     const int rowbase = tm * BM + wm * WM;            // band n1
-    const int colbase = mt * BN + wn * WN;            // my_igp within frequency f
+    const int colbase = (KSPLIT ? mt / a.ksplit : mt) * BN + wn * WN;   // my_igp within frequency f
     const int kl = lane >> 4, cl = lane & 15;
 
     d4 acc_re[MB][NB], acc_im[MB][NB];
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
     for (int j = 0; j < NB; ++j)
         pb[j] = a.X + ((size_t)f * a.G + min(colbase + j * 16 + cl, a.G - 1)) * a.C;
     auto load = [&](Frags<MB, NB> &fr, int s) {
-        s = min(s, a.nks - 1);
+        s = min(s, s_end - 1);
         const int k = 4 * s + kl;
         const int kc = min(k, a.C - 1);
         const bool kok = k < a.C;
@@ -154,8 +161,8 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void ff_fused_kernel(FfArgs a
     };
     Frags<MB, NB> ring[P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) load(ring[q], q);
-    for (int s0 = 0; s0 < a.nks; s0 += P) {
+    for (int q = 0; q < P; ++q) load(ring[q], s_beg + q);
+    for (int s0 = s_beg; s0 < s_end; s0 += P) {
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             mma(ring[q]);
@@ -423,10 +430,17 @@ constexpr int kNumFfCfgs = sizeof(kFfCfgs) / sizeof(kFfCfgs[0]);
 
 void launch_ff_fused(int cfg, int tiles, hipStream_t s, const FfArgs &a)
 {
+    const bool ks = a.ksplit > 1;
     switch (cfg) {
-    case 0: hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2>), dim3(tiles), dim3(256), 0, s, a); break;
+    case 0:
+        if (ks) hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2, true>), dim3(tiles), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((ff_fused_kernel<2, 2, 3, 2, 2, 2, false>), dim3(tiles), dim3(256), 0, s, a);
+        break;
     case 2: hipLaunchKernelGGL(ff_fewband_kernel, dim3(tiles), dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1>), dim3(tiles), dim3(256), 0, s, a); break;
+    default:
+        if (ks) hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1, true>), dim3(tiles), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((ff_fused_kernel<2, 4, 3, 2, 2, 1, false>), dim3(tiles), dim3(256), 0, s, a);
+        break;
     }
 }
 
@@ -456,6 +470,19 @@ FfLayout ff_plan(const fk_ff_problem *p, const fk_options *o, Scratch &ws)
             const int tiles = L.tiles_m * p->nfreq * L.mtiles, chunks = L.nks / 4;
             while (tiles * L.ksplit < 1024 && L.ksplit < 8 && chunks / (2 * L.ksplit) >= 2) L.ksplit *= 2;
             L.mtiles *= L.ksplit;
+        } else {
+            // too few tiles for two rounds over the CUs (small problems,
+            // the per-band row contraction): split the ig range, each part
+            // at least 16 ring passes, no part empty
+            const int tiles = L.tiles_m * p->nfreq * L.mtiles, rounds = L.nks / c.p;
+            int ks = 1;
+            while (tiles * ks < 512 * c.minw && ks < 8 && rounds >= 2 * ks * 16) ks *= 2;
+            if (ks > 1) {
+                const int per = div_up(rounds, ks);
+                L.ksplit = div_up(rounds, per);
+                L.kper = per * c.p;
+                L.mtiles *= L.ksplit;
+            }
         }
     }
     L.off_cf = ws.plan(sizeof(double2) * (size_t)std::max(p->number_bands, 1) * p->ngpown);
@@ -505,9 +532,10 @@ static int ff_fp64_gemm(const FfLayout &L, const double2 *aqsn, const double2 *X
     a.nb = nb; a.C = C; a.G = G; a.nfreq = nf; a.nks = L.nks;
     a.tiles_m = L.tiles_m; a.mtiles = L.mtiles; a.tiles_n = nf * L.mtiles;
     a.ksplit = L.ksplit;
+    a.kper = L.kper;
     const int tiles = a.tiles_m * a.tiles_n;
     a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
-    if (kFfCfgs[L.cfg].minw == 2 && L.cfg != kFfConfigFewBands - 1) {   // MFMA-bound tiles only
+    if (kFfCfgs[L.cfg].minw == 2 && L.cfg != kFfConfigFewBands - 1 && L.ksplit == 1) {   // MFMA-bound tiles only
         int dev = 0, ncu = 0;
         FK_HIP(hipGetDevice(&dev));
         FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));

@@ -136,7 +136,8 @@ int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, co
 constexpr int kFfConfigFewBands = 3;
 struct FfLayout {
     int cfg, tiles_m, mtiles, nks;
-    int ksplit = 1;         // the few-band kernel's ig parts per column block
+    int ksplit = 1;         // ig parts per column block (few tiles, or the few-band kernel)
+    int kper = 0;           // k-steps per part (the FP64 fused tiles with ksplit > 1)
     bool split;             // FK_PREC_MIXED: split BF16 (fk_ff_split.hip)
     FfSplitLayout sp;
     size_t off_cf, off_vg, off_igp, off_part, off_achs;

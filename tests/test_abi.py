@@ -163,6 +163,20 @@ def test_batch_rejects_host_memory_and_bad_sizes(fk, fko):
     assert not h.value
 
 
+def test_ff_workspace_grows_with_ig_split(fk):
+    """Few FP64 tiles and a long ig range split the range into parts with a
+    partial each: 70 bands x 40 columns x 2 frequencies at ncouls 3072 takes
+    8 parts (64x64 tiles: 2 x 1 x 2 tiles; 256 ring passes of 3 k-steps), so
+    the partials grow 8x over ncouls 256, whose 22 passes are too few to
+    split (a part keeps at least 16); nothing else in the workspace depends
+    on ncouls."""
+    nb, G, nf = 70, 40, 2
+    part = 2 * 16 * nf * nb          # bytes of partials per column tile
+    short = fk.ff_workspace_size(nb, 256, G, nf, config=1)
+    long_ = fk.ff_workspace_size(nb, 3072, G, nf, config=1)
+    assert abs((long_ - short) - 7 * part) <= 256        # pieces are 256-byte aligned
+
+
 def test_ff_full_workspace_and_checks(fk, fko):
     """ff_kernel_full's workspace covers the I_epsR contraction and, for the
     bands and frequencies the SX / delta terms read, the I_epsA one; its

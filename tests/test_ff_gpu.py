@@ -114,6 +114,23 @@ def test_tile_configs(fk, fko, config):
     _check(fko, fk, p, config=config)
 
 
+@pytest.mark.parametrize("config", [1, 2])
+@pytest.mark.parametrize("shape", [(70, 3072, 40, 2), (20, 2000, 33, 3), (5, 1100, 17, 1)])
+def test_ig_split_small_problems(fk, fko, config, shape):
+    """Few tiles and a long ig range: the FP64 tiles split the ig range over
+    up to 8 workgroups (8, 4 and 4 parts here, the last one shorter), whose
+    partials the reduction sums; with skipped and permuted columns."""
+    nb, C, G, nf = shape
+    print(f"ff ig split config {config} shape {shape}", flush=True)
+    p = fko.ff_random(nb, C, G, nf, seed=7 + nb)
+    inv = p.inv_igp_index.copy()
+    inv[[0, 2]] = inv[[2, 0]]
+    ind = p.indinv.copy()
+    ind[inv[4]] = -1
+    p.inv_igp_index, p.indinv = inv, ind
+    _check(fko, fk, p, config=config)
+
+
 def test_device_stream_workspace_timer_and_no_schDv(fk, fko):
     torch = _torch()
     p = fko.ff_random(40, 200, 50, 3, seed=5)
