@@ -1,7 +1,7 @@
 """Write the golden vectors under tests/golden/ from the compiled reference.
 
-Run in a container that has the reference checkout (its sources are compiled
-by `make -C oracle ref` into oracle/_ref/); the GPU box only reads the
+Run where the reference checkout exists (its sources are compiled by
+`make -C oracle ref` into oracle/_ref/); the GPU tests only read the
 fixtures this script writes.
 
 Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
