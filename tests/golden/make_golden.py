@@ -29,7 +29,8 @@ Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
                    achDtemp_cor_Kernel (FFKer_cori/ffKernel.cpp:10-300) called
                    in the order of its main, one thread
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [ff_full case ...]
+(with case names, only those ff_full_<case>.npz fixtures are written)
 """
 from __future__ import annotations
 
@@ -218,7 +219,7 @@ def run_ff_full_harness(p: fko.FfFullProblem, schDt):
     return out
 
 
-def ff_full_cases():
+def ff_full_cases(only=None):
     """Seeded inputs for the C++ port's FF kernel functions: random band
     energies (both signs of wx for occupied bands), a non-uniform grid, an
     evaluation axis whose freqevalmin / freqevalstep the port truncates to
@@ -235,7 +236,20 @@ def ff_full_cases():
     cases["seeded"] = (p, (rng.uniform(-1, 1, (p.nb, p.nfreq)) + 1j * rng.uniform(-1, 1, (p.nb, p.nfreq))) * 50)
     q = fko.ff_full_random(12, 3, 24, 6, 5, 9, seed=42)     # the program's axes, schDt from the contraction
     cases["program_axes"] = (q, fko.ff_full(q)["schDt"])
+    # a negative freqevalmin, which the port truncates toward zero (int(-1.75)
+    # = -1), a non-uniform grid, schDt from the contraction
+    rng2 = np.random.default_rng(2025)
+    r = fko.ff_full_random(13, 4, 36, 7, 6, 8, seed=43)
+    r.ekq = rng2.uniform(-12.0, 12.0, r.nb)
+    r.dFreqGrid = np.cumsum(rng2.uniform(0.5, 3.0, r.nfreq)) - 0.4
+    r.dFreqGrid[0] = 0.0
+    r.pref = rng2.uniform(0.1, 0.6, r.nfreq)
+    r.dFreqBrd = rng2.uniform(-0.05, 0.05, r.nfreq) + 1j * rng2.uniform(0.05, 0.2, r.nfreq)
+    r.freqevalmin, r.freqevalstep, r.occ = -1.75, 2.5, 0.5
+    cases["negative_axes"] = (r, fko.ff_full(r)["schDt"])
     for name, (p, schDt) in cases.items():
+        if only and name not in only:
+            continue
         out = run_ff_full_harness(p, schDt)
         np.savez_compressed(os.path.join(HERE, f"ff_full_{name}.npz"),
                             sizes=np.array([p.nb, p.nv, p.ncouls, p.ngpown, p.nfreq, p.nfreqeval], np.int64),
@@ -268,6 +282,9 @@ def main():
     for exe in ("ref_gpp_harness", "ref_ff_harness", "gppKer", "ffKernel"):
         if not os.path.exists(os.path.join(REF_BIN, exe)):
             sys.exit(f"missing oracle/_ref/{exe}: run `make -C oracle ref` where the reference is present")
+    if len(sys.argv) > 1:      # only the named ff_full cases
+        ff_full_cases(only=set(sys.argv[1:]))
+        return
     save_gpp("random", case_random())
     save_gpp("nearpole", case_nearpole())
     save_gpp("branch3", case_branch3())

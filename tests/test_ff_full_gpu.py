@@ -248,13 +248,16 @@ def _fixture(fko, name):
     return p, d
 
 
-def test_cpp_port_matches_reference_functions(fk, fko):
-    """ff_full_program_axes.npz holds what the C++ port's own kernel functions
+@pytest.mark.parametrize("name", ["program_axes", "negative_axes"])
+def test_cpp_port_matches_reference_functions(fk, fko, name):
+    """ff_full_<name>.npz holds what the C++ port's own kernel functions
     (compiled from FFKer_cori/ffKernel.cpp, one thread) return for these
     inputs with schDt_matrix = the contraction of I_epsR, which is what the
-    device evaluates with under FK_FF_CPP_PORT."""
-    p, d = _fixture(fko, "program_axes")
-    print("ff_full cpp port vs reference functions, program axes", flush=True)
+    device evaluates with under FK_FF_CPP_PORT: the program's axes, and a
+    non-uniform grid with freqevalmin -1.75, freqevalstep 2.5 (which the port
+    truncates to -1 and 2)."""
+    p, d = _fixture(fko, name)
+    print(f"ff_full cpp port vs reference functions, {name}", flush=True)
     got = _np(fk.ff_full_problem(_to_dev(p), semantics=fk.FK_FF_CPP_PORT))
     for k in OUTS:
         ref = d[k]

@@ -40,7 +40,7 @@ def _load(fko, name):
     return p, d
 
 
-@pytest.mark.parametrize("name", ["seeded", "program_axes"])
+@pytest.mark.parametrize("name", ["seeded", "program_axes", "negative_axes"])
 def test_cpp_variant_matches_reference_functions(fko, name):
     p, d = _load(fko, name)
     o = fko.ff_full(p, variant=1, schDt_in=d["schDt"])
