@@ -268,11 +268,14 @@ def test_cpp_port_matches_reference_functions(fk, fko, name):
     assert abs(got["achsDtemp"] - complex(d["achsDtemp"])) <= RTOL * abs(complex(d["achsDtemp"]))
 
 
-@pytest.mark.parametrize("case", [("seeded", 0), ("seeded", 1), ("variety", 3), ("variety", 4)])
+@pytest.mark.parametrize("case", [("seeded", 0), ("seeded", 1), ("variety", 3), ("variety", 4), ("variety", 5),
+                                  ("variety", 6)])
 def test_cpp_port_vs_oracle(fk, fko, case):
     """The port's arithmetic on inputs that exercise it: a non-uniform grid,
     band energies that put wx' on both sides of zero, freqevalmin and
-    freqevalstep that the port truncates to int (0.25, 1.5).  Against the
+    freqevalstep that the port truncates to int (0.25, 1.5), every band
+    occupied with two grid frequencies, no band occupied with one evaluation
+    frequency.  Against the
     oracle's variant 1 (pinned to the compiled port) with the oracle's own
     contraction as schDt_matrix (FK_FF_CPP_PORT) and with zeros
     (FK_FF_CPP_PORT_ZERO_SCHDT, the C++ program as it runs)."""
