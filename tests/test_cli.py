@@ -174,6 +174,25 @@ def test_ffker_random_matches_oracle(fko):
 
 
 @pytest.mark.gpu
+def test_ffker_random_port_semantics_and_bad_value(fko):
+    """--semantics 1 on the seeded inputs: every printed output equals the
+    oracle's variant 1 with the oracle's own contraction as schDt_matrix;
+    an unknown semantics value is the library's argument error."""
+    nb, nv, C, G, nf, nfe = 16, 4, 24, 8, 6, 10
+    r = _run("ffKer", nb, nv, C, G, nf, nfe, "--random", 5, "--semantics", 1)
+    assert r.returncode == 0, r.stderr
+    p = fko.ff_full_random(nb, nv, C, G, nf, nfe, seed=5)
+    p.achs_freq = 1
+    want = fko.ff_full(p, variant=1, schDt_in=fko.ff_full(p)["schDt"])
+    for name in fko.FF_OUTPUTS:
+        got = np.array(_complex_lines(r.stdout, name))
+        scale = max(np.max(np.abs(want[name])), 1e-300)
+        assert np.max(np.abs(got - want[name])) <= 1e-12 * scale, name
+    bad = _run("ffKer", nb, nv, C, G, nf, nfe, "--semantics", 7)
+    assert bad.returncode != 0 and "semantics" in bad.stderr
+
+
+@pytest.mark.gpu
 def test_ffker_contraction_mode(fko):
     d = np.load(os.path.join(GOLDEN, "ff_main.npz"))
     (nb, nv, C, G, nf, nfe), ref = d["shapes"][0], d["achsDtemp"][0]
