@@ -150,6 +150,21 @@ def test_gpp_frequency_counts(fk, fko, nw):
     _assert_close(asx, ref_asx, what=f"nw={nw}")
 
 
+@pytest.mark.parametrize("consts", [(1e-4, 0.5, 1e3, 0.5), (1e-8, 0.04, 1e7, 2.0)])
+def test_gpp_thresholds_and_evaluation_axis(fk, fko, consts):
+    """Other to1 / limittwo / limitone / occ than the program's, and an
+    evaluation axis with a negative frequency and points inside the wtilde
+    range (near-pole terms in every branch)."""
+    p = fko.gpp_random(48, 5, 640, 37, nw=4, seed=77, wtilde_re=(0.5, 5.0))
+    p.to1, p.limittwo, p.limitone, p.occ = consts
+    p.wx = np.array([-1.3, 0.7, 2.9, 4.4])
+    print(f"gpp thresholds {consts}", flush=True)
+    ref_ach, ref_asx, _ = fko.gpp(p)
+    ach, asx, _ = _run_dev(fk, p)
+    _assert_close(ach, ref_ach, what=f"achtemp {consts}")
+    _assert_close(asx, ref_asx, what=f"asxtemp {consts}")
+
+
 def test_gpp_mask_off_semantics(fk, fko):
     d = np.load(os.path.join(GOLDEN, "gpp_nearpole.npz"))
     p = _from_npz(fko, d)

@@ -67,6 +67,21 @@ def test_mixed_frequency_counts(fk, fko, nw):
     assert _rel(ach, ra) <= MIXED_RTOL and _rel(asx, rs) <= MIXED_RTOL
 
 
+@pytest.mark.parametrize("consts", [(1e-4, 0.5, 1e3, 0.5), (1e-8, 0.04, 1e7, 2.0)])
+def test_mixed_thresholds_and_evaluation_axis(fk, fko, consts):
+    """The int8-split path with other thresholds and an evaluation axis with
+    a negative frequency and points inside the wtilde range."""
+    p = _round64(fko.gpp_random(48, 5, 640, 37, nw=4, seed=77, wtilde_re=(0.5, 5.0)))
+    p.to1, p.limittwo, p.limitone, p.occ = consts
+    p.wx = np.array([-1.3, 0.7, 2.9, 4.4])
+    print(f"mixed thresholds {consts}", flush=True)
+    ra, rs, _ = fko.gpp(p)
+    ach, asx = _run(fk, p, to1=p.to1, limittwo=p.limittwo, limitone=p.limitone, occ=p.occ)
+    ea, es = _rel(ach, ra), _rel(asx, rs)
+    print(f"  errors {ea:.2e} {es:.2e}", flush=True)
+    assert ea <= MIXED_RTOL and es <= MIXED_RTOL
+
+
 def test_mixed_host_memory_mask_off_and_skipped_columns(fk, fko):
     p = _round64(fko.gpp_random(24, 2, 512, 40, seed=8, wtilde_re=(3.0, 5.0)))
     ind = p.indinv.copy()
