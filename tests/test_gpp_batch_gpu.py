@@ -8,9 +8,10 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-12
 
-# (nb, nv, ncouls, ngpown, nfreq): uneven shapes, no occupied band, 11 frequencies
+# (nb, nv, ncouls, ngpown, nfreq): uneven shapes, no occupied band, 11 and
+# 16 (the most) frequencies
 SHAPES = [(40, 2, 512, 25, 3), (7, 3, 100, 9, 5), (64, 0, 300, 30, 4), (33, 16, 256, 12, 11), (1, 1, 64, 3, 3),
-          (96, 8, 1024, 51, 7)]
+          (96, 8, 1024, 51, 7), (20, 4, 200, 10, 16)]
 
 
 def _torch():
