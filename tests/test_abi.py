@@ -243,3 +243,22 @@ def test_ff_batch_rejects_host_memory_and_bad_config(fk, fko):
     opt.config = 9
     assert fk.lib.ff_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_ARG
     assert not h.value
+
+
+def test_package_refuses_to_import_without_the_library(tmp_path):
+    """No fallback: a copy of the package whose lib/ holds no built library
+    raises ImportError on import instead of computing anything elsewhere."""
+    import importlib.util
+    import os
+    import shutil
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fortkernels-_amd",
+                       "__init__.py")
+    pkg = tmp_path / "fk_copy"
+    pkg.mkdir()
+    shutil.copy(src, pkg / "__init__.py")
+    spec = importlib.util.spec_from_file_location("fk_copy_without_lib", str(pkg / "__init__.py"),
+                                                  submodule_search_locations=[str(pkg)])
+    mod = importlib.util.module_from_spec(spec)
+    with pytest.raises(ImportError) as e:
+        spec.loader.exec_module(mod)
+    assert "shared library not built" in str(e.value)
