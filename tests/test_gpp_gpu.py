@@ -150,17 +150,19 @@ def test_gpp_frequency_counts(fk, fko, nw):
     _assert_close(asx, ref_asx, what=f"nw={nw}")
 
 
+@pytest.mark.parametrize("mask", [True, False])
 @pytest.mark.parametrize("consts", [(1e-4, 0.5, 1e3, 0.5), (1e-8, 0.04, 1e7, 2.0)])
-def test_gpp_thresholds_and_evaluation_axis(fk, fko, consts):
+def test_gpp_thresholds_and_evaluation_axis(fk, fko, consts, mask):
     """Other to1 / limittwo / limitone / occ than the program's, and an
     evaluation axis with a negative frequency and points inside the wtilde
-    range (near-pole terms in every branch)."""
+    range (near-pole terms in every branch), with the canonical mask and
+    without it (the GPU variant's achtemp)."""
     p = fko.gpp_random(48, 5, 640, 37, nw=4, seed=77, wtilde_re=(0.5, 5.0))
     p.to1, p.limittwo, p.limitone, p.occ = consts
     p.wx = np.array([-1.3, 0.7, 2.9, 4.4])
-    print(f"gpp thresholds {consts}", flush=True)
-    ref_ach, ref_asx, _ = fko.gpp(p)
-    ach, asx, _ = _run_dev(fk, p)
+    print(f"gpp thresholds {consts} mask {mask}", flush=True)
+    ref_ach, ref_asx, _ = fko.gpp(p, mask_on=mask)
+    ach, asx, _ = _run_dev(fk, p, mask=mask)
     _assert_close(ach, ref_ach, what=f"achtemp {consts}")
     _assert_close(asx, ref_asx, what=f"asxtemp {consts}")
 
