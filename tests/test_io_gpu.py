@@ -91,6 +91,18 @@ def test_ff_instance_cli(fk, fko, tmp_path):
     b = _run("ffKer", "--input", dump)
     for name in fko.FF_OUTPUTS:
         assert np.array_equal(_lines(a, name), _lines(b, name)), name
+    # the C++ port's semantics on the same instance (the file holds inputs;
+    # the evaluation is chosen on the command line)
+    out = _run("ffKer", "--input", path, "--semantics", 1)
+    q = fko.ff_full_random(30, 6, 90, 20, 7, 9, seed=14)
+    q.freqevalmin = 0.35
+    ref1 = fko.ff_full(q, variant=1, schDt_in=ref["schDt"])
+    for name in fko.FF_OUTPUTS:
+        got = _lines(out, name)
+        if np.max(np.abs(ref1[name])) == 0:
+            assert np.max(np.abs(got)) == 0, name
+        else:
+            assert _rel(got, ref1[name]) <= 1e-12, name
 
 
 @pytest.mark.parametrize("complex64", [False, True])
