@@ -37,7 +37,7 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "gpp_kernel_ex", "fk_gpp_instance_write", "fk_ff_instance_write", "fk_instance_open",
            "fk_instance_kind", "fk_instance_complex_bytes", "fk_instance_gpp", "fk_instance_ff", "fk_instance_close",
            "fk_instance_list_read", "fk_instance_list_free", "ff_batch_create", "ff_batch_run", "ff_batch_config",
-           "ff_batch_workspace_bytes", "ff_batch_destroy"]
+           "ff_batch_workspace_bytes", "ff_batch_destroy", "gpp_kernel_noalloc", "gpp_kernel_noalloc_max_workspace"]
 
 
 class FkError(RuntimeError):
@@ -114,6 +114,10 @@ def _load():
     L.gpp_kernel.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), P, P, P]
     L.gpp_kernel_workspace_size.restype = SZ
     L.gpp_kernel_workspace_size.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options)]
+    L.gpp_kernel_noalloc.restype = ctypes.c_int
+    L.gpp_kernel_noalloc.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), P, P, P]
+    L.gpp_kernel_noalloc_max_workspace.restype = SZ
+    L.gpp_kernel_noalloc_max_workspace.argtypes = []
     L.ff_kernel.restype = ctypes.c_int
     L.ff_kernel.argtypes = [ctypes.POINTER(FfProblem), ctypes.POINTER(Options), P, P, P]
     L.ff_kernel_workspace_size.restype = SZ

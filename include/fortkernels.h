@@ -164,6 +164,23 @@ typedef struct {
 
 int gpp_kernel_ex(const fk_gpp_problem *problem, const fk_options *options, const fk_gpp_outputs *outputs);
 
+/* gpp_kernel that allocates nothing.  No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * workspace is the library's own device array of
+ * gpp_kernel_noalloc_max_workspace() bytes (64 MiB), one per GPU, held until
+ * the process ends; the call calls no hipMalloc, hipFree or allocating
+ * library routine.  Same arguments, precisions, configurations and results
+ * as gpp_kernel, limited to: options->memory = FK_MEM_DEVICE,
+ * options->workspace = NULL, options->kernel_ms = NULL, acht_n1_loc = NULL,
+ * and problems whose gpp_kernel_workspace_size() fits the array (else
+ * FK_ERR_WORKSPACE; 512/2/32768/20 in FP64 needs 15 MB).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+int gpp_kernel_noalloc(const fk_gpp_problem *problem, const fk_options *options,
+                       fk_cdouble *achtemp, fk_cdouble *asxtemp, fk_cdouble *acht_n1_loc);
+size_t gpp_kernel_noalloc_max_workspace(void);
+
 /* ------------------------------------------------------------------------
  * Batched GPP: many independent problems in one launch.  No counterpart in
  * the reference, whose program runs one problem's band loop
