@@ -38,6 +38,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -205,15 +206,23 @@ int fk_instance_open(const char *path, fk_instance **out)
     // expected extents of the arrays, in file order
     std::vector<size_t> ext;
     const size_t e = (size_t)h.complex_bytes;
+    // a * b * c bytes, or SIZE_MAX when the product does not fit (then no
+    // file is large enough and the extent check below rejects it)
+    auto prod = [](size_t a, size_t b, size_t c) -> size_t {
+        size_t ab, abc;
+        if (__builtin_mul_overflow(a, b, &ab) || __builtin_mul_overflow(ab, c, &abc)) return SIZE_MAX;
+        return abc;
+    };
     if (h.kind == 1) {
         const size_t nb = h.sizes[0], C = h.sizes[2], G = h.sizes[3], nf = h.sizes[4];
         if (C < 1 || G < 1 || nf < 1 || h.sizes[1] > h.sizes[0]) return fail("bad GPP sizes");
-        ext = {8 * nf, 4 * G, 4 * (C + 1), 8 * C, e * nb * C, e * nb * C, e * G * C, e * G * C};
+        ext = {8 * nf, 4 * G, 4 * (C + 1), 8 * C, prod(e, nb, C), prod(e, nb, C), prod(e, G, C), prod(e, G, C)};
     } else {
         const size_t nb = h.sizes[0], C = h.sizes[2], G = h.sizes[3], nf = h.sizes[4];
         if (nb < 1 || C < 1 || G < 1 || nf < 1 || h.sizes[5] < 1 || h.sizes[1] > h.sizes[0]) return fail("bad FF sizes");
-        ext = {8 * nb, 8 * nf, 8 * nf, 16 * nf, 4 * G, 4 * (C + 1), 8 * C, 16 * nb * C, 16 * nb * C,
-               16 * nf * G * C, 16 * nf * G * C};
+        const size_t fgc = prod(nf, G, C);
+        ext = {8 * nb, 8 * nf, 8 * nf, 16 * nf, 4 * G, 4 * (C + 1), 8 * C, prod(16, nb, C), prod(16, nb, C),
+               prod(16, fgc, 1), prod(16, fgc, 1)};
     }
     std::vector<const void *> ptr(ext.size());
     for (size_t i = 0; i < ext.size(); ++i) {

@@ -110,3 +110,34 @@ def test_instance_list(fk, tmp_path):
     empty = sub / "empty.txt"
     empty.write_text("# nothing\n")
     assert fk.instance_list_read(str(empty)) == []
+
+
+def test_header_whose_extents_wrap_is_rejected(fk, tmp_path):
+    """number_bands = ncouls = 2^30 make 16 * nb * ncouls = 2^64, which wraps
+    to 0 in 64-bit arithmetic; every other extent fits a 48 GiB sparse file,
+    so only an overflow-checked extent rejects the header."""
+    nb = C = 1 << 30
+    path = str(tmp_path / "wrap.fki")
+    size = 48 << 30
+    offs = [256, 320, 384]                                  # wx, inv_igp_index, indinv
+    offs.append(offs[-1] + (4 * (C + 1) + 63) // 64 * 64)  # vcoul
+    offs.append(offs[-1] + 8 * C)                           # aqsmtemp (its wrapped extent would be 0)
+    offs.append(offs[-1])                                   # aqsntemp
+    offs.append(offs[-1])                                   # I_eps_array: 16 * ngpown * C = 16 GiB
+    offs.append(offs[-1] + 16 * C)                          # wtilde_array
+    assert offs[-1] + 16 * C <= size
+    header = HEADER.pack(b"FKINST01", 1, 16, nb, 0, C, 1, 1, 0, 0, 0, 1e-6, 0.25, 2.5e5, 1.0, 0, 0, 0, 0,
+                         *(offs + [0] * 8))
+    try:
+        with open(path, "wb") as f:
+            f.write(header)
+            f.truncate(size)
+    except OSError as e:
+        pytest.skip(f"no sparse file of {size} bytes here: {e}")
+    with pytest.raises(fk.FkError) as e:
+        fk.Instance(path)
+    assert e.value.code == fk.FK_ERR_ARG
+    msg = fk.last_error()
+    if "mmap" in msg:
+        pytest.skip(f"cannot map a {size}-byte file here: {msg}")
+    assert "extent" in msg
