@@ -60,7 +60,11 @@ class GppProblem(ctypes.Structure):
                 ("wtilde_array", ctypes.c_void_p), ("vcoul", ctypes.c_void_p),
                 ("inv_igp_index", ctypes.c_void_p), ("indinv", ctypes.c_void_p),
                 ("to1", ctypes.c_double), ("limittwo", ctypes.c_double), ("limitone", ctypes.c_double),
-                ("occ", ctypes.c_double)]
+                ("occ", ctypes.c_double), ("semantics", ctypes.c_int32)]
+
+
+# fk_gpp_semantics
+FK_GPP_CANONICAL, FK_GPP_GPPSUM = 0, 1
 
 
 class FfProblem(ctypes.Structure):
@@ -214,10 +218,85 @@ def _ptr(x):
     if isinstance(x, int):
         return x
     if _is_torch(x):
-        assert x.is_contiguous(), "tensor must be contiguous"
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
         return x.data_ptr()
-    assert isinstance(x, np.ndarray) and x.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    if not (isinstance(x, np.ndarray) and x.flags["C_CONTIGUOUS"]):
+        raise ValueError("array must be a C-contiguous numpy array")
     return x.ctypes.data
+
+
+_PRECISIONS = {"fp64": FK_PREC_FP64, "mixed": FK_PREC_MIXED}
+_NP = {"c128": np.complex128, "c64": np.complex64, "f64": np.float64}
+
+
+def _precision(precision):
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+    return _PRECISIONS[precision]
+
+
+def _bulk(where, arrays):
+    """Check the bulk arrays of one call before the library sees them.
+
+    arrays: (name, x, kind, numel) with kind "c128" / "c64" / "f64" and numel
+    the number of elements the kernels read.  Every array must be a
+    C-contiguous numpy array (host memory) or every one a contiguous torch
+    tensor on the current ROCm device (device memory), of exactly that
+    element type and with at least numel elements.  Returns True for device
+    memory.  Raises TypeError / ValueError, so a wrong buffer never reaches
+    a kernel."""
+    kinds = {_is_torch(x) for _, x, _, _ in arrays if x is not None}
+    if len(kinds) != 1:
+        raise TypeError(f"{where}: bulk arrays must be all numpy (host) or all torch device tensors")
+    on_dev = kinds.pop()
+    dev = None
+    if on_dev:
+        import torch
+        tdt = {"c128": torch.complex128, "c64": torch.complex64, "f64": torch.float64}
+        dev = torch.cuda.current_device() if torch.cuda.is_available() else None
+    for name, x, kind, numel in arrays:
+        if x is None:
+            raise ValueError(f"{where}: {name} is missing")
+        if on_dev:
+            if not x.is_cuda:
+                raise TypeError(f"{where}: {name} is a CPU tensor; pass numpy arrays for host memory")
+            if x.device.index != dev:
+                raise ValueError(f"{where}: {name} is on {x.device}, the library runs on cuda:{dev}")
+            if x.dtype != tdt[kind]:
+                raise TypeError(f"{where}: {name} must be {tdt[kind]}, not {x.dtype}")
+            if not x.is_contiguous():
+                raise ValueError(f"{where}: {name} must be contiguous")
+            n = x.numel()
+        else:
+            if not isinstance(x, np.ndarray):
+                raise TypeError(f"{where}: {name} must be a numpy array or a torch device tensor")
+            if x.dtype != _NP[kind]:
+                raise TypeError(f"{where}: {name} must be {np.dtype(_NP[kind])}, not {x.dtype}")
+            if not x.flags["C_CONTIGUOUS"]:
+                raise ValueError(f"{where}: {name} must be C-contiguous")
+            n = x.size
+        if n < numel:
+            raise ValueError(f"{where}: {name} has {n} elements, the problem reads {numel}")
+    return on_dev
+
+
+def _host_len(where, name, a, n):
+    if a.shape[0] < n:
+        raise ValueError(f"{where}: {name} has {a.shape[0]} entries, the problem needs {n}")
+
+
+def _workspace(where, workspace, on_dev):
+    """(pointer, bytes) of a caller workspace: a device tensor for device
+    memory, on the current device."""
+    if workspace is None:
+        return None, 0
+    if not _is_torch(workspace) or not workspace.is_cuda:
+        raise TypeError(f"{where}: the workspace must be a torch device tensor")
+    import torch
+    if workspace.device.index != torch.cuda.current_device():
+        raise ValueError(f"{where}: the workspace is on {workspace.device}")
+    return _ptr(workspace), int(workspace.numel() * workspace.element_size())
 
 
 def _stream_handle(stream):
@@ -238,11 +317,13 @@ def default_options() -> Options:
 def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv_igp_index, indinv, *,
                to1=1e-6, limittwo=0.25, limitone=2.5e5, occ=1.0, precision="fp64", mask=True,
                stream=None, workspace=None, config=0, time_kernel=False, device=-1, acht_n1=False,
-               achstemp=False):
+               achstemp=False, semantics=FK_GPP_CANONICAL):
     """Run gpp_kernel.  Returns (achtemp, asxtemp, kernel_ms|None), and with
     acht_n1=True (achtemp, asxtemp, kernel_ms|None, acht_n1_loc[nb]).
     achstemp=True (through gpp_kernel_ex) appends the static-CH remainder
     achstemp (a one-element array / tensor) as the last item.
+    semantics=FK_GPP_GPPSUM evaluates the gppsum = 1 loop of gppKerSeq.cpp
+    (FP64 only; achtemp and asxtemp).
 
     numpy inputs -> host memory mode, outputs are numpy complex128 arrays.
     torch device tensors -> device mode, outputs are torch complex128 device
@@ -251,19 +332,23 @@ def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv
     inv = np.ascontiguousarray(inv_igp_index, np.int32)
     ind = np.ascontiguousarray(indinv, np.int32)
     nw = int(wx.shape[0])
-    on_dev = _is_torch(aqsn)
+    prec = _precision(precision)
+    ck = "c64" if prec == FK_PREC_MIXED else "c128"
+    nbC, GC = int(nb) * int(ncouls), int(ngpown) * int(ncouls)
+    on_dev = _bulk("gpp_kernel", [("aqsm", aqsm, ck, nbC), ("aqsn", aqsn, ck, nbC), ("I_eps", I_eps, ck, GC),
+                                  ("wtilde", wtilde, ck, GC), ("vcoul", vcoul, "f64", int(ncouls))])
+    _host_len("gpp_kernel", "inv_igp_index", inv, int(ngpown))
+    _host_len("gpp_kernel", "indinv", ind, int(ncouls) + 1)
     prob = GppProblem(int(nb), int(nv), int(ncouls), int(ngpown), nw, wx.ctypes.data, _ptr(aqsm), _ptr(aqsn),
                       _ptr(I_eps), _ptr(wtilde), _ptr(vcoul), inv.ctypes.data, ind.ctypes.data,
-                      float(to1), float(limittwo), float(limitone), float(occ))
+                      float(to1), float(limittwo), float(limitone), float(occ), int(semantics))
     opt = default_options()
     opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = prec
     opt.mask = 1 if mask else 0
     opt.device = int(device)
     opt.stream = _stream_handle(stream)
-    if workspace is not None:
-        opt.workspace = _ptr(workspace)
-        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.workspace, opt.workspace_bytes = _workspace("gpp_kernel", workspace, on_dev)
     opt.config = int(config)
     ms = ctypes.c_float(-1.0)
     if time_kernel:
@@ -298,7 +383,7 @@ def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0
     prob = GppProblem(nb, nv, ncouls, ngpown, nfreq, wx.ctypes.data, dummy, dummy, dummy, dummy, dummy,
                       inv.ctypes.data, ind.ctypes.data, 1e-6, 0.25, 2.5e5, 1.0)
     opt = default_options()
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = _precision(precision)
     opt.config = config
     return int(lib.gpp_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
 
@@ -324,9 +409,18 @@ class FfBatch:
     def __init__(self, problems, precision="fp64", config=0):
         self._keep = []
         structs = []
-        for p in problems:
+        prec = _precision(precision)
+        for k, p in enumerate(problems):
             inv = np.ascontiguousarray(p["inv_igp_index"], np.int32)
             ind = np.ascontiguousarray(p["indinv"], np.int32)
+            where = f"FfBatch problem {k}"
+            nbC = int(p["nb"]) * int(p["ncouls"])
+            if not _bulk(where, [("aqsm", p["aqsm"], "c128", nbC), ("aqsn", p["aqsn"], "c128", nbC),
+                                 ("I_eps", p["I_eps"], "c128", int(p["nfreq"]) * int(p["ngpown"]) * int(p["ncouls"])),
+                                 ("vcoul", p["vcoul"], "f64", int(p["ncouls"]))]):
+                raise TypeError(f"{where}: batches take torch device tensors")
+            _host_len(where, "inv_igp_index", inv, int(p["ngpown"]))
+            _host_len(where, "indinv", ind, int(p["ncouls"]) + 1)
             self._keep += [inv, ind, p]
             structs.append(FfProblem(int(p["nb"]), int(p["ncouls"]), int(p["ngpown"]), int(p["nfreq"]),
                                      _ptr(p["aqsm"]), _ptr(p["aqsn"]), _ptr(p["I_eps"]), _ptr(p["vcoul"]),
@@ -337,7 +431,7 @@ class FfBatch:
         arr = (ctypes.POINTER(FfProblem) * self.n)(*[ctypes.pointer(s) for s in structs])
         opt = default_options()
         opt.memory = FK_MEM_DEVICE
-        opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+        opt.precision = prec
         opt.config = int(config)
         h = ctypes.c_void_p()
         _check(lib.ff_batch_create(self.n, arr, ctypes.byref(opt), ctypes.byref(h)))
@@ -386,10 +480,20 @@ class GppBatch:
     def __init__(self, problems, precision="fp64", config=0, mask=True):
         self._keep = []
         structs = []
-        for p in problems:
+        prec = _precision(precision)
+        ck = "c64" if prec == FK_PREC_MIXED else "c128"
+        for k, p in enumerate(problems):
             wx = np.ascontiguousarray(p["wx"], np.float64)
             inv = np.ascontiguousarray(p["inv_igp_index"], np.int32)
             ind = np.ascontiguousarray(p["indinv"], np.int32)
+            where = f"GppBatch problem {k}"
+            nbC, GC = int(p["nb"]) * int(p["ncouls"]), int(p["ngpown"]) * int(p["ncouls"])
+            if not _bulk(where, [("aqsm", p["aqsm"], ck, nbC), ("aqsn", p["aqsn"], ck, nbC),
+                                 ("I_eps", p["I_eps"], ck, GC), ("wtilde", p["wtilde"], ck, GC),
+                                 ("vcoul", p["vcoul"], "f64", int(p["ncouls"]))]):
+                raise TypeError(f"{where}: batches take torch device tensors")
+            _host_len(where, "inv_igp_index", inv, int(p["ngpown"]))
+            _host_len(where, "indinv", ind, int(p["ncouls"]) + 1)
             self._keep += [wx, inv, ind, p]
             structs.append(GppProblem(int(p["nb"]), int(p["nv"]), int(p["ncouls"]), int(p["ngpown"]), int(wx.shape[0]),
                                       wx.ctypes.data, _ptr(p["aqsm"]), _ptr(p["aqsn"]), _ptr(p["I_eps"]),
@@ -402,7 +506,7 @@ class GppBatch:
         arr = (ctypes.POINTER(GppProblem) * self.n)(*[ctypes.pointer(s) for s in structs])
         opt = default_options()
         opt.memory = FK_MEM_DEVICE
-        opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+        opt.precision = prec
         opt.mask = 1 if mask else 0
         opt.config = int(config)
         h = ctypes.c_void_p()
@@ -455,17 +559,21 @@ def ff_kernel(nb, ncouls, ngpown, nfreq, aqsm, aqsn, I_eps, vcoul, inv_igp_index
     in split BF16 on MFMA with FP32 accumulation and FP64 refinement."""
     inv = np.ascontiguousarray(inv_igp_index, np.int32)
     ind = np.ascontiguousarray(indinv, np.int32)
-    on_dev = _is_torch(aqsn)
+    prec = _precision(precision)
+    nbC = int(nb) * int(ncouls)
+    on_dev = _bulk("ff_kernel", [("aqsm", aqsm, "c128", nbC), ("aqsn", aqsn, "c128", nbC),
+                                 ("I_eps", I_eps, "c128", int(nfreq) * int(ngpown) * int(ncouls)),
+                                 ("vcoul", vcoul, "f64", int(ncouls))])
+    _host_len("ff_kernel", "inv_igp_index", inv, int(ngpown))
+    _host_len("ff_kernel", "indinv", ind, int(ncouls) + 1)
     prob = FfProblem(int(nb), int(ncouls), int(ngpown), int(nfreq), _ptr(aqsm), _ptr(aqsn), _ptr(I_eps),
                      _ptr(vcoul), inv.ctypes.data, ind.ctypes.data, int(achs_freq))
     opt = default_options()
     opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = prec
     opt.device = int(device)
     opt.stream = _stream_handle(stream)
-    if workspace is not None:
-        opt.workspace = _ptr(workspace)
-        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.workspace, opt.workspace_bytes = _workspace("ff_kernel", workspace, on_dev)
     opt.config = int(config)
     ms = ctypes.c_float(-1.0)
     if time_kernel:
@@ -488,7 +596,7 @@ def ff_workspace_size(nb, ncouls, ngpown, nfreq, config=0, precision="fp64"):
     ind = np.zeros(ncouls + 1, np.int32)
     prob = FfProblem(nb, ncouls, ngpown, nfreq, dummy, dummy, dummy, dummy, inv.ctypes.data, ind.ctypes.data, 0)
     opt = default_options()
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = _precision(precision)
     opt.config = config
     return int(lib.ff_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
 
@@ -518,17 +626,22 @@ def ff_kernel_full(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR,
     host = [np.ascontiguousarray(inv_igp_index, np.int32), np.ascontiguousarray(indinv, np.int32),
             np.ascontiguousarray(ekq, np.float64), np.ascontiguousarray(dFreqGrid, np.float64),
             np.ascontiguousarray(pref, np.float64), np.ascontiguousarray(dFreqBrd, np.complex128)]
-    on_dev = _is_torch(aqsn)
+    prec = _precision(precision)
+    nbC, fGC = int(nb) * int(ncouls), int(nfreq) * int(ngpown) * int(ncouls)
+    on_dev = _bulk("ff_kernel_full", [("aqsm", aqsm, "c128", nbC), ("aqsn", aqsn, "c128", nbC),
+                                      ("I_epsR", I_epsR, "c128", fGC), ("I_epsA", I_epsA, "c128", fGC),
+                                      ("vcoul", vcoul, "f64", int(ncouls))])
+    for name, a, n in zip(("inv_igp_index", "indinv", "ekq", "dFreqGrid", "pref", "dFreqBrd"), host,
+                          (ngpown, ncouls + 1, nb, nfreq, nfreq, nfreq)):
+        _host_len("ff_kernel_full", name, a, int(n))
     prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, aqsm, aqsn, I_epsR, I_epsA, vcoul, host,
                            (pref_zb, freqevalmin, freqevalstep, occ, achs_freq, semantics))
     opt = default_options()
     opt.memory = FK_MEM_DEVICE if on_dev else FK_MEM_HOST
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = prec
     opt.device = int(device)
     opt.stream = _stream_handle(stream)
-    if workspace is not None:
-        opt.workspace = _ptr(workspace)
-        opt.workspace_bytes = int(workspace.numel() * workspace.element_size()) if _is_torch(workspace) else workspace.nbytes
+    opt.workspace, opt.workspace_bytes = _workspace("ff_kernel_full", workspace, on_dev)
     opt.config = int(config)
     ms = ctypes.c_float(-1.0)
     if time_kernel:
@@ -558,7 +671,7 @@ def ff_full_workspace_size(nb, nv, ncouls, ngpown, nfreq, nfreqeval, ekq, dFreqG
     prob = _ff_full_struct(nb, nv, ncouls, ngpown, nfreq, nfreqeval, dummy, dummy, dummy, dummy, dummy, host,
                            (0.0, freqevalmin, freqevalstep, 1.0, 0, semantics))
     opt = default_options()
-    opt.precision = FK_PREC_MIXED if precision == "mixed" else FK_PREC_FP64
+    opt.precision = _precision(precision)
     opt.config = config
     return int(lib.ff_kernel_full_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
 
@@ -684,12 +797,18 @@ def instance_list_read(path):
 def fill_complex(dst, seed, stream_id, re, im, stream=None):
     """Counter-based fill of a complex128 or complex64 device tensor."""
     import torch
+    if not (_is_torch(dst) and dst.is_cuda and dst.is_contiguous()
+            and dst.dtype in (torch.complex128, torch.complex64)):
+        raise TypeError("fill_complex: dst must be a contiguous complex128 / complex64 device tensor")
     as64 = 1 if dst.dtype == torch.complex64 else 0
     _check(lib.fk_fill_complex(_ptr(dst), dst.numel(), seed, stream_id, re[0], re[1], im[0], im[1], as64,
                                _stream_handle(stream)))
 
 
 def fill_real(dst, seed, stream_id, lo, hi, stream=None):
+    import torch
+    if not (_is_torch(dst) and dst.is_cuda and dst.is_contiguous() and dst.dtype == torch.float64):
+        raise TypeError("fill_real: dst must be a contiguous float64 device tensor")
     _check(lib.fk_fill_real(_ptr(dst), dst.numel(), seed, stream_id, lo, hi, _stream_handle(stream)))
 
 

@@ -597,12 +597,14 @@ int choose_cfg(const fk_gpp_problem *p, const fk_options *o)
 struct Layout {
     int cfg, Kpad, nvpad, ldc, tiles_m, tiles_n;
     size_t off_c, off_igp, off_part, off_achs;
+    size_t off_stop;   // the gppsum = 1 form: the occupied loop's stop per column
 };
 
 Layout plan_layout(const fk_gpp_problem *p, const fk_options *o, Scratch &ws, size_t elem)
 {
     Layout L;
-    L.cfg = choose_cfg(p, o);
+    const bool sym = p->semantics == FK_GPP_GPPSUM;
+    L.cfg = sym ? 1 : choose_cfg(p, o);   // the gppsum = 1 kernel has 64 x 64 tiles
     const TileCfg &t = kCfgs[L.cfg];
     L.nvpad = div_up(p->nvband, 4) * 4;
     const int rows = std::max(L.nvpad + p->number_bands - p->nvband, 1);
@@ -614,6 +616,7 @@ Layout plan_layout(const fk_gpp_problem *p, const fk_options *o, Scratch &ws, si
     L.off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * L.tiles_m * L.tiles_n);
     L.off_achs = ws.plan(sizeof(double2) * (size_t)L.tiles_m * L.tiles_n);   // used when achstemp is asked for
+    L.off_stop = sym ? ws.plan(sizeof(int) * (size_t)p->ngpown) : 0;
     return L;
 }
 
@@ -636,7 +639,59 @@ int check_problem(const fk_gpp_problem *p, const fk_options *o)
     if (o->precision != FK_PREC_FP64 && o->precision != FK_PREC_MIXED) { set_error("gpp_kernel: bad precision"); return FK_ERR_ARG; }
     if (o->config < 0 || o->config > kNumCfgs) { set_error("gpp_kernel: config %d out of range", o->config); return FK_ERR_ARG; }
     if ((int64_t)p->ngpown * p->ncouls > (int64_t)INT32_MAX * 8) { set_error("gpp_kernel: problem too large"); return FK_ERR_ARG; }
+    if (p->semantics != FK_GPP_CANONICAL && p->semantics != FK_GPP_GPPSUM) {
+        set_error("gpp_kernel: unknown semantics %d", p->semantics);
+        return FK_ERR_ARG;
+    }
+    if (p->semantics == FK_GPP_GPPSUM && (o->precision != FK_PREC_FP64 || o->mask != 1 || o->config != 0)) {
+        set_error("gpp_kernel: the gppsum = 1 form runs in FP64 with the default mask and configuration only");
+        return FK_ERR_UNSUPPORTED;
+    }
     return FK_OK;
+}
+
+// c[Kpad][ldc] and the device copy of the column map (gpp_coef_kernel, in
+// chunks of kColChunk columns whose map travels in the kernel arguments).
+template <class E>
+int launch_coef(const fk_gpp_problem *p, const Layout &L, const std::vector<int32_t> &cols, const E *aqsm,
+                const double *vcoul, E *c, int *igp, hipStream_t s)
+{
+    for (int m_begin = 0; m_begin < L.ldc; m_begin += kColChunk) {
+        CoefArgs<E> ca;
+        ca.aqsm = aqsm; ca.vcoul = vcoul; ca.c = c; ca.igp_out = igp;
+        ca.nb = p->number_bands; ca.nv = p->nvband; ca.nvpad = L.nvpad;
+        ca.C = p->ncouls; ca.G = p->ngpown; ca.Kpad = L.Kpad; ca.ldc = L.ldc;
+        ca.m_begin = m_begin;
+        ca.m_count = std::min(kColChunk, L.ldc - m_begin);
+        for (int mm = 0; mm < kColChunk; ++mm) {
+            int m = m_begin + mm;
+            ca.igp[mm] = (mm < ca.m_count && m < p->ngpown) ? cols[m] : -1;
+        }
+        int64_t n = (int64_t)L.Kpad * ca.m_count;
+        int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(gpp_coef_kernel<E>, dim3(blocks), dim3(256), 0, s, ca);
+        FK_HIP(hipGetLastError());
+    }
+    return FK_OK;
+}
+
+// The gppsum = 1 form (fk_gpp_sym.hip) on FP64 inputs in device memory.
+int launch_gpp_sym(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Scratch &ws,
+                   const std::vector<int32_t> &cols, const double2 *aqsm, const double2 *aqsn, const double2 *ieps,
+                   const double2 *wt, const double *vcoul, double2 *d_ach, double2 *d_asx, hipStream_t s)
+{
+    if (p->number_bands == 0) {
+        hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, d_ach, d_asx, p->nfreq, nullptr);
+        FK_HIP(hipGetLastError());
+        return FK_OK;
+    }
+    double2 *c = static_cast<double2 *>(ws.at(L.off_c));
+    int *igp = static_cast<int *>(ws.at(L.off_igp));
+    double2 *part = static_cast<double2 *>(ws.at(L.off_part));
+    FK_TRY(launch_coef<double2>(p, L, cols, aqsm, vcoul, c, igp, s));
+    FK_TRY(gpp_sym_launch(p, c, L.ldc, L.Kpad, L.nvpad, igp, static_cast<int *>(ws.at(L.off_stop)), aqsm, aqsn,
+                          ieps, wt, part, L.tiles_m, L.tiles_n, o->kernel_ms, s));
+    return gpp_reduce_launch(part, L.tiles_m * L.tiles_n, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s);
 }
 
 template <class E>
@@ -653,23 +708,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     E *c = static_cast<E *>(ws.at(L.off_c));
     int *igp = static_cast<int *>(ws.at(L.off_igp));
     double2 *part = static_cast<double2 *>(ws.at(L.off_part));
-
-    for (int m_begin = 0; m_begin < L.ldc; m_begin += kColChunk) {
-        CoefArgs<E> ca;
-        ca.aqsm = aqsm; ca.vcoul = vcoul; ca.c = c; ca.igp_out = igp;
-        ca.nb = p->number_bands; ca.nv = p->nvband; ca.nvpad = L.nvpad;
-        ca.C = C; ca.G = G; ca.Kpad = L.Kpad; ca.ldc = L.ldc;
-        ca.m_begin = m_begin;
-        ca.m_count = std::min(kColChunk, L.ldc - m_begin);
-        for (int mm = 0; mm < kColChunk; ++mm) {
-            int m = m_begin + mm;
-            ca.igp[mm] = (mm < ca.m_count && m < G) ? cols[m] : -1;
-        }
-        int64_t n = (int64_t)L.Kpad * ca.m_count;
-        int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
-        hipLaunchKernelGGL(gpp_coef_kernel<E>, dim3(blocks), dim3(256), 0, s, ca);
-        FK_HIP(hipGetLastError());
-    }
+    FK_TRY(launch_coef<E>(p, L, cols, aqsm, vcoul, c, igp, s));
 
     GppArgs<E> a;
     a.c = c; a.aqsn = aqsn; a.I_eps = ieps; a.wtilde = wt; a.igp = igp; a.partials = part;
@@ -1431,6 +1470,11 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
         set_error("gpp_kernel: acht_n1_loc needs nfreq >= 3 (got %d)", p->nfreq);
         return FK_ERR_ARG;
     }
+    const bool sym = p->semantics == FK_GPP_GPPSUM;
+    if (sym && (acht_n1_loc || out->achstemp)) {
+        set_error("gpp_kernel: the gppsum = 1 form gives achtemp and asxtemp only (no acht_n1_loc, no achstemp)");
+        return FK_ERR_UNSUPPORTED;
+    }
     std::vector<int32_t> igp;
     FK_TRY(resolve_columns(p->ncouls, p->ngpown, p->inv_igp_index, p->indinv, igp));
 
@@ -1493,8 +1537,12 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
                 } else {
                     const double2 *m = (const double2 *)p->aqsmtemp, *n = (const double2 *)p->aqsntemp;
                     const double2 *ie = (const double2 *)p->I_eps_array, *w = (const double2 *)p->wtilde_array;
-                    rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
-                    if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                    if (sym) {
+                        rc = launch_gpp_sym(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
+                    } else {
+                        rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
+                        if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                    }
                 }
             } else if (mixed) {
                 rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, s,
@@ -1507,6 +1555,7 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
                 rc = run_host<double2>(p, o, igp, ach, asx, n1, achs, s,
                                        [&](const double2 *m, const double2 *n, const double2 *ie, const double2 *w,
                                            const double *v, double2 *da, double2 *dx, double2 *dc) {
+                                           if (sym) return launch_gpp_sym(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);
                                            return launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, dc,
                                                                       s);
                                        });
@@ -1547,6 +1596,11 @@ int create_batch(int32_t n, const fk_gpp_problem *const *problems, const fk_opti
     }
     for (int i = 0; i < n; ++i) {
         if (!problems[i]) { set_error("gpp_batch_create: problem %d is NULL", i); return FK_ERR_ARG; }
+        if (problems[i]->semantics != FK_GPP_CANONICAL) {
+            set_error("gpp_batch_create: problem %d: batches run the canonical semantics (call gpp_kernel for "
+                      "the gppsum = 1 form)", i);
+            return FK_ERR_UNSUPPORTED;
+        }
         fk_options oi = *options;
         oi.config = 0;
         const int rc = check_problem(problems[i], &oi);

@@ -98,6 +98,15 @@ int gpp_sum_launch(const double2 *partials, int n, double2 *out, hipStream_t s);
 int gpp_reduce_launch(const double2 *partials, int tiles, int nw, double asx_scale, double2 *achtemp,
                       double2 *asxtemp, hipStream_t s);
 
+// The gppsum = 1 form of the GPP band loop (fk_gpp_sym.hip): the occupied
+// loop's stop per column (when nvband > 0) into d_stop, then the tile kernel
+// over c[Kpad][ldc] (the canonical coefficient matrix) writing per-tile
+// partials [tiles_m * tiles_n][nfreq][2] for gpp_reduce_launch.
+// kernel_ms (optional) times the tile kernel.
+int gpp_sym_launch(const fk_gpp_problem *p, const double2 *c, int ldc, int Kpad, int nvpad, const int *d_igp,
+                   int *d_stop, const double2 *aqsm, const double2 *aqsn, const double2 *ieps, const double2 *wt,
+                   double2 *partials, int tiles_m, int tiles_n, float *kernel_ms, hipStream_t s);
+
 // Mixed-precision GPP on int8 MFMA (complex64 inputs): workspace layout,
 // size limits, and the launch of its three kernels plus the reduction.
 struct OzLayout {

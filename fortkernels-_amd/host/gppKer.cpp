@@ -2,6 +2,12 @@
 // with the band loop replaced by one gpp_kernel() call on the MI355X.
 //
 //   gppKer <number_bands> <number_valence_bands> <number_plane_waves> <nodes_per_mpi_group> [options]
+//   gppKer <number_bands> <number_valence_bands> <number_plane_waves> <nodes_per_mpi_group> 1 [options]
+//
+// The five-number form is the command line of the sequential program
+// gppKerSeq.cpp (:264-276) with gppsum = 1: the same inputs, its band loop
+// (FK_GPP_GPPSUM semantics, FP64) and the same output lines.  Only gppsum = 1
+// is built; the four-number form runs BGW-Kernels/gppKer.cpp's loop.
 //
 // Without options it builds the reference's inputs (every aqsmtemp, aqsntemp,
 // I_eps_array and wtilde_array entry 0.5+0.5i, vcoul 1, inv_igp_index[i] =
@@ -17,7 +23,8 @@
 //                   inv_igp_index[i] = i*ncouls/ngpown, indinv[ncouls] = ncouls-1
 //   --near-pole     with --random: wtilde re ~ U(3.0,5.0)
 //   --nfreq N       number of frequencies (default 3; wx[iw] = 3 + iw)
-//   --mixed         complex64 storage, FP32 band contraction (FK_PREC_MIXED)
+//   --mixed         complex64 storage, FK_PREC_MIXED: the band contraction on
+//                   int8 MFMA by the exact-accumulation split (config 0)
 //   --unmasked      the GPU variants' unmasked sum (gppKer_gpuComplex.cpp:171-178)
 //   --acht-n1       also compute and print acht_n1_loc[n1] (:424)
 //   --achstemp      also compute and print the static-CH remainder achstemp of
@@ -50,7 +57,7 @@ void usage()
 {
     std::cout << "The correct form of input is : " << std::endl;
     std::cout << " ./a.out <number_bands> <number_valence_bands> <number_plane_waves> <nodes_per_mpi_group> "
-                 "[--random SEED] [--near-pole] [--nfreq N] [--mixed] [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
+                 "[gppsum=1] [--random SEED] [--near-pole] [--nfreq N] [--mixed] [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
                  "[--device D] [--dump FILE]\n ./a.out --input FILE [--unmasked] [--acht-n1] [--achstemp] [--repeat K] "
                  "[--device D]"
               << std::endl;
@@ -71,7 +78,7 @@ void hip_check(hipError_t e, const char *what)
 }
 
 struct Args {
-    int nb = 0, nv = 0, ncouls = 0, npg = 0;
+    int nb = 0, nv = 0, ncouls = 0, npg = 0, gppsum = -1;   // -1: the four-number form
     bool random = false, near_pole = false, mixed = false, unmasked = false, acht_n1 = false, achstemp = false;
     unsigned long long seed = 0;
     int nfreq = 3, repeat = 1, device = -1;
@@ -100,7 +107,8 @@ bool parse(int argc, char **argv, Args &a)
         else return false;
     }
     if (!a.input.empty()) return pos.empty() && a.repeat >= 1 && !a.random && a.dump.empty();
-    if (pos.size() != 4) return false;
+    if (pos.size() != 4 && pos.size() != 5) return false;
+    if (pos.size() == 5) a.gppsum = std::atoi(pos[4].c_str());
     a.nb = std::atoi(pos[0].c_str());
     a.nv = std::atoi(pos[1].c_str());
     a.ncouls = std::atoi(pos[2].c_str());
@@ -231,6 +239,12 @@ int main(int argc, char **argv)
         return 0;
     }
     if (!a.input.empty()) return run_input(a);
+    if (a.gppsum >= 0 && (a.gppsum != 1 || a.mixed || a.unmasked || a.acht_n1 || a.achstemp)) {
+        std::cerr << "gppKer: the five-number form (gppKerSeq.cpp) is built for gppsum = 1, in FP64, without "
+                     "--mixed, --unmasked, --acht-n1 or --achstemp; the four-number form runs "
+                     "BGW-Kernels/gppKer.cpp's loop" << std::endl;
+        return 2;
+    }
     if (a.nb < 1 || a.nv < 0 || a.nv > a.nb || a.ncouls < 1 || a.npg < 1 || a.npg > a.ncouls) {
         std::cerr << "gppKer: need number_bands >= 1, 0 <= number_valence_bands <= number_bands, "
                      "1 <= nodes_per_mpi_group <= number_plane_waves" << std::endl;
@@ -324,6 +338,7 @@ int main(int argc, char **argv)
     p.aqsmtemp = pm; p.aqsntemp = pn; p.I_eps_array = pe; p.wtilde_array = pw; p.vcoul = pv;
     p.inv_igp_index = inv.data(); p.indinv = indinv.data();
     p.to1 = to1; p.limittwo = limittwo; p.limitone = limitone; p.occ = 1.0;
+    p.semantics = a.gppsum == 1 ? FK_GPP_GPPSUM : FK_GPP_CANONICAL;
 
     if (!a.dump.empty()) dump_instance(a.dump, p, a.mixed, opt.memory == FK_MEM_DEVICE);
     run_and_print(p, opt, a);

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define FK_ABI_VERSION 1
+#define FK_ABI_VERSION 2
 
 typedef struct { double re, im; } fk_cdouble;  /* complex128 */
 typedef struct { float re, im; } fk_cfloat;    /* complex64 (mixed-precision storage) */
@@ -131,7 +131,34 @@ typedef struct {
     double limittwo;             /* 0.25 */
     double limitone;             /* 1/(4 to1) */
     double occ;                  /* 1.0 */
+    int32_t semantics;           /* fk_gpp_semantics; 0 (zero-initialised) = canonical */
 } fk_gpp_problem;
+
+/* Which program's band loop gpp_kernel evaluates.
+ * FK_GPP_CANONICAL: BGW-Kernels/gppKer.cpp:303-439 (the formulas above).
+ * FK_GPP_GPPSUM: the sequential program gppKerSeq.cpp run with its fifth
+ *   argument gppsum = 1 (:76-261, 383-472).  With igp the column's G-vector:
+ *   every ig sum runs over ig <= igp and adds beside each term
+ *   m1 x (m1 = aqsn[n1][ig] conj(aqsm[n1][igp])) its mirror m2 conj(x)
+ *   (m2 = conj(aqsm[n1][ig]) aqsm[n1][igp]);
+ *   occupied bands (flagOCC_solver :90-139): ig < igp up to the first ig
+ *     with |wtilde^2 I_eps| < to1; delw = (wx - wtilde) (0.5/wtilde); where
+ *     |wx - wtilde| < sqrt(limittwo) or |delw|^2 < to1: sch = 0,
+ *     ssx = -Omega2 / (4 wtilde^2 (1 + delw)) (0 if |wtilde| <= to1); else
+ *     sch = wtilde I_eps / (wx - wtilde), ssx = Omega2 / (wx^2 - wtilde^2);
+ *     no mirror at ig = igp - 1;
+ *   unoccupied bands (noflagOCC_solver :206-242 over 512-wide blocks):
+ *     ig < igp - 1 except ig = 511 mod 512, plus ig = igp (m1 term only)
+ *     when igp is a positive multiple of 512; sch = delw I_eps with
+ *     delw = wtilde / (wx - wtilde), kept only where |wdiff|^2 < limittwo
+ *     or |delw|^2 > limitone;
+ *   achtemp[iw] = sum 0.5 vcoul[igp] scht, asxtemp[iw] = sum occ vcoul[igp] ssxt.
+ *   FP64 only (precision FK_PREC_FP64, mask 1, config 0), no acht_n1_loc or
+ *   achstemp, not in batches (FK_ERR_UNSUPPORTED otherwise). */
+typedef enum {
+    FK_GPP_CANONICAL = 0,
+    FK_GPP_GPPSUM = 1
+} fk_gpp_semantics;
 
 /* Replaces BGW-Kernels/gppKer.cpp:303-439 (and gppKer_gpuComplex.cpp:148
  * achtemp_kernel + :127 tillNvband_kernel).  achtemp, asxtemp: [nfreq].

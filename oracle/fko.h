@@ -20,6 +20,10 @@
  *                      ssxt_scht_solver on seeded inputs), brute-force
  *                      branch cases.
  *   fko_gpp_point    : golden vectors of the compiled ssxt_scht_solver.
+ *   fko_gpp_gppsum   : golden vectors of the compiled gppKerSeq.cpp
+ *                      flagOCC_solver / noflagOCC_solver (gppsum = 1) and
+ *                      of its program's printed achtemp; the closed form of
+ *                      constant inputs; brute force of the block bounds.
  *   fko_gpp_achstemp : the compiled achsDtemp_Kernel (FFKer_cori/ffKernel.cpp:
  *                      107-131), the same sum, run on the GPP inputs
  *                      (golden vectors); closed form of constant inputs.
@@ -94,6 +98,27 @@ int fko_gpp(int nb, int nv, int ncouls, int ngpown, int nw,
 int fko_gpp_achstemp(int nb, int ncouls, int ngpown, const fko_z *aqsm, const fko_z *aqsn,
                      const fko_z *I_eps, const double *vcoul, const int32_t *inv_igp_index,
                      const int32_t *indinv, int nthreads, fko_z *achstemp);
+
+/* The gppsum = 1 form of the band loop, gppKerSeq.cpp:76-261,383-472 (the
+ * reference's sequential program with its fifth argument gppsum = 1): the
+ * ig sum runs over ig < igp with the occupied bands' break at the first
+ * |wtilde^2 I_eps| < to1, the mirrored term m2 conj(x) beside every m1 x
+ * (m2 = conj(aqsm[ig]) aqsm[igp]), the unoccupied bands keep only their
+ * near-pole terms and skip ig = 511 mod 512 and igp - 1 (block bounds),
+ * with the diagonal ig = igp when igp is a positive multiple of 512.  Sums
+ * are long double in a fixed order.  acht_n1_loc may be NULL (needs nw >= 3).
+ * Returns 0, or -1 on bad sizes / an out-of-range index map. */
+int fko_gpp_gppsum(int nb, int nv, int ncouls, int ngpown, int nw,
+                   const double *wx, const fko_z *aqsm, const fko_z *aqsn,
+                   const fko_z *I_eps, const fko_z *wtilde, const double *vcoul,
+                   const int32_t *inv_igp_index, const int32_t *indinv,
+                   double to1, double limittwo, double limitone, double occ,
+                   int nthreads, fko_z *achtemp, fko_z *asxtemp, fko_z *acht_n1_loc);
+
+/* Which ig enter the unoccupied gppsum = 1 sum of column igp: 1 = both
+ * terms, 2 = the diagonal (m1 term only), 0 = none (gppKerSeq.cpp:209-241
+ * with the 512-wide blocks of :438-450). */
+int fko_gppsum_unocc_kind(int ig, int igp);
 
 /* Resolve the column map exactly as BGW-Kernels/gppKer.cpp:322-328:
  * igp_out[my_igp] = igp, or -1 when the reference skips the column.

@@ -54,6 +54,10 @@ def lib() -> ctypes.CDLL:
         L.fko_resolve_igp.argtypes = [I, I, P, P, P]
         L.fko_ff.restype = I
         L.fko_ff.argtypes = [I, I, I, I] + [P] * 6 + [I, I] + [P] * 3
+        L.fko_gpp_gppsum.restype = I
+        L.fko_gpp_gppsum.argtypes = [I, I, I, I, I] + [P] * 8 + [D, D, D, D] + [I] + [P] * 3
+        L.fko_gppsum_unocc_kind.restype = I
+        L.fko_gppsum_unocc_kind.argtypes = [I, I]
         L.fko_gpp_achstemp.restype = I
         L.fko_gpp_achstemp.argtypes = [I, I, I] + [P] * 6 + [I, P]
         L.fko_ff_full.restype = I
@@ -200,6 +204,27 @@ def gpp(p: GppProblem, mask_on=True, extended=True, nthreads=0, band_lo=0, band_
     if rc != 0:
         raise ValueError("oracle rejected the GPP problem")
     return ach, asx, (n1loc[: p.nb] if want_n1loc else None)
+
+
+def gpp_gppsum(p: GppProblem, nthreads=0, want_n1loc=False):
+    """The gppsum = 1 band loop of gppKerSeq.cpp.  Returns (achtemp[nw],
+    asxtemp[nw], acht_n1_loc[nb] or None)."""
+    ach = np.zeros(p.nw, np.complex128)
+    asx = np.zeros(p.nw, np.complex128)
+    n1loc = np.zeros(max(p.nb, 1), np.complex128) if want_n1loc else None
+    arrs = [_c(p.wx, np.float64), _c(p.aqsm, np.complex128), _c(p.aqsn, np.complex128),
+            _c(p.I_eps, np.complex128), _c(p.wtilde, np.complex128), _c(p.vcoul, np.float64),
+            _c(p.inv_igp_index, np.int32), _c(p.indinv, np.int32)]
+    rc = lib().fko_gpp_gppsum(p.nb, p.nv, p.ncouls, p.ngpown, p.nw, *[_p(a) for a in arrs],
+                              p.to1, p.limittwo, p.limitone, p.occ, int(nthreads),
+                              _p(ach), _p(asx), _p(n1loc) if want_n1loc else None)
+    if rc != 0:
+        raise ValueError("oracle rejected the GPP problem")
+    return ach, asx, (n1loc[: p.nb] if want_n1loc else None)
+
+
+def gppsum_unocc_kind(ig, igp):
+    return int(lib().fko_gppsum_unocc_kind(int(ig), int(igp)))
 
 
 def gpp_point(wxt, wtilde, I_eps, to1=TO1, limittwo=LIMITTWO, limitone=LIMITONE):

@@ -29,8 +29,16 @@ Fixtures (numpy .npz, no pickles; inputs and expected outputs only):
                    achDtemp_cor_Kernel (FFKer_cori/ffKernel.cpp:10-300) called
                    in the order of its main, one thread
 
-Usage: python tests/golden/make_golden.py [ff_full case ...]
-(with case names, only those ff_full_<case>.npz fixtures are written)
+  gppsum_<case>.npz  inputs + achtemp/asxtemp/acht_n1_loc of the gppsum = 1
+                   band loop, written by oracle/_ref/ref_gppseq_harness, i.e.
+                   by the reference's flagOCC_solver / noflagOCC_solver of
+                   gppKerSeq.cpp (:76-261) called as its main calls them
+  gppsum_main.npz  achtemp printed by the reference's gppKerSeq program
+                   (gppKerSeq.cpp main, constant inputs, gppsum = 1)
+
+Usage: python tests/golden/make_golden.py [gppsum | ff_full case ...]
+(with `gppsum`, only the gppsum fixtures; with ff_full case names, only
+those ff_full_<case>.npz fixtures are written)
 """
 from __future__ import annotations
 
@@ -278,10 +286,83 @@ def gpp_achstemp_case():
     print("gpp_achstemp:", achs)
 
 
+def run_gppseq_harness(p: fko.GppProblem, gppsum=1):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([p.nb, p.nv, p.ncouls, p.ngpown, p.nw, gppsum], np.int32).tofile(f)
+            np.array([p.occ], np.float64).tofile(f)
+            np.ascontiguousarray(p.wx, np.float64).tofile(f)
+            for a in (p.aqsm, p.aqsn, p.I_eps, p.wtilde):
+                np.ascontiguousarray(a, np.complex128).tofile(f)
+            np.ascontiguousarray(p.vcoul, np.float64).tofile(f)
+            np.ascontiguousarray(p.inv_igp_index, np.int32).tofile(f)
+            np.ascontiguousarray(p.indinv, np.int32).tofile(f)
+        subprocess.run([os.path.join(REF_BIN, "ref_gppseq_harness"), fin, fout], check=True)
+        out = np.fromfile(fout, np.complex128)
+    nw, nb = p.nw, p.nb
+    assert out.size == 2 * nw + nb
+    return out[:nw], out[nw:2 * nw], out[2 * nw:]
+
+
+def gppsum_cases():
+    """Seeded problems for the gppsum = 1 loop: wtilde near the frequencies
+    (so the unoccupied bands' near-pole terms are kept), ig ranges that
+    cross the 512-wide blocks, columns at igp = 0, 1, 511, 512, 513, 1024
+    (the diagonal term), the indigp == ncouls clamp, a skipped column, and
+    occupied-band loops that stop early (|wtilde^2 I_eps| < to1)."""
+    cases = {}
+    cases["random"] = fko.gpp_random(6, 2, 1100, 11, nw=3, seed=41, wtilde_re=(2.5, 5.5))
+    p = fko.gpp_random(5, 2, 1300, 12, nw=4, seed=42, wtilde_re=(2.5, 5.5))
+    p.inv_igp_index = np.array([0, 1, 2, 511, 512, 513, 700, 1023, 1024, 1025, 1299, 1300], np.int32)
+    ind = np.arange(1301, dtype=np.int32)
+    ind[1300] = 1299
+    ind[700] = -3            # skipped column
+    p.indinv = ind
+    # near-pole wtilde (|wx - wtilde| < 0.5 for wx = 3) at the points the
+    # block bounds decide: the diagonal terms (ig = igp = 512, 1024), the
+    # dropped ig = igp - 1 and ig = 511, and the first kept neighbours
+    w = p.wtilde.copy()
+    for m, ig in ((4, 512), (8, 1024), (4, 511), (8, 1023), (5, 511), (5, 512), (9, 1023), (9, 1022),
+                  (10, 1298), (10, 1297), (7, 510), (7, 1022)):
+        w[m, ig] = 3.1 + 0.02j
+    p.wtilde = w
+    cases["blocks"] = p
+    q = fko.gpp_random(4, 3, 900, 9, nw=3, seed=43, wtilde_re=(2.5, 5.5))
+    ie = q.I_eps.copy()
+    for m, ig in ((2, 5), (4, 130), (6, 400), (7, 0), (8, 799)):
+        ie[m, ig] = 1e-9 + 2e-9j     # |wtilde^2 I_eps| < to1: the occupied loop stops here
+    q.I_eps = ie
+    cases["break"] = q
+    for name, p in cases.items():
+        ach, asx, n1loc = run_gppseq_harness(p)
+        np.savez_compressed(
+            os.path.join(HERE, f"gppsum_{name}.npz"),
+            nb=p.nb, nv=p.nv, ncouls=p.ncouls, ngpown=p.ngpown, occ=p.occ,
+            wx=p.wx, aqsm=p.aqsm, aqsn=p.aqsn, I_eps=p.I_eps, wtilde=p.wtilde, vcoul=p.vcoul,
+            inv_igp_index=p.inv_igp_index, indinv=p.indinv,
+            achtemp=ach, asxtemp=asx, acht_n1_loc=n1loc)
+        print(f"gppsum_{name}: achtemp={ach} asxtemp={asx}")
+    shapes = [(8, 2, 1100, 10), (6, 3, 1536, 12), (4, 4, 600, 6)]
+    rows = []
+    for nb, nv, C, npg in shapes:
+        out = subprocess.run([os.path.join(REF_BIN, "gppKerSeq"), str(nb), str(nv), str(C), str(npg), "1"],
+                             check=True, capture_output=True, text=True).stdout
+        vals = re.findall(r"achtemp\[(\d)\] = \(([-\d.e+]+),([-\d.e+]+)\)", out)
+        assert len(vals) == 3, out
+        rows.append([complex(float(a), float(b)) for _, a, b in vals])
+    np.savez_compressed(os.path.join(HERE, "gppsum_main.npz"), shapes=np.array(shapes, np.int64),
+                        achtemp=np.array(rows))
+    print("gppsum_main:", rows)
+
+
 def main():
-    for exe in ("ref_gpp_harness", "ref_ff_harness", "gppKer", "ffKernel"):
+    for exe in ("ref_gpp_harness", "ref_gppseq_harness", "ref_ff_harness", "gppKer", "gppKerSeq", "ffKernel"):
         if not os.path.exists(os.path.join(REF_BIN, exe)):
             sys.exit(f"missing oracle/_ref/{exe}: run `make -C oracle ref` where the reference is present")
+    if sys.argv[1:] == ["gppsum"]:
+        gppsum_cases()
+        return
     if len(sys.argv) > 1:      # only the named ff_full cases
         ff_full_cases(only=set(sys.argv[1:]))
         return
@@ -294,6 +375,7 @@ def main():
     ff_cases()
     ff_full_cases()
     gpp_achstemp_case()
+    gppsum_cases()
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol(fk):
 
 
 def test_abi_version_and_target(fk):
-    assert fk.lib.fk_abi_version() == 1
+    assert fk.lib.fk_abi_version() == 2
     assert fk.lib.fk_build_target() == b"gfx950"
 
 
@@ -106,7 +106,7 @@ def test_ff_rejects_bad_problems_before_touching_a_device(fk, fko):
     p = fko.ff_random(4, 32, 8, 2, seed=1)
     args = (p.aqsm, p.aqsn, p.I_eps, p.vcoul, p.inv_igp_index, p.indinv)
     with pytest.raises(fk.FkError) as e:
-        fk.ff_kernel(p.nb, 8, 32, p.nfreq, *args)
+        fk.ff_kernel(p.nb, 8, 32, p.nfreq, p.aqsm, p.aqsn, p.I_eps, p.vcoul, np.zeros(32, np.int32), p.indinv)
     assert e.value.code == fk.FK_ERR_ARG and "cannot be greater" in fk.last_error()
     for kw in ({"achs_freq": 2}, {"achs_freq": -1}):
         with pytest.raises(fk.FkError) as e:
@@ -262,3 +262,75 @@ def test_package_refuses_to_import_without_the_library(tmp_path):
     with pytest.raises(ImportError) as e:
         spec.loader.exec_module(mod)
     assert "shared library not built" in str(e.value)
+
+
+def test_gppsum_form_rejections_before_any_device_work(fk, fko):
+    """FK_GPP_GPPSUM runs FP64 with the default mask and configuration and
+    gives achtemp / asxtemp only; everything else is refused on the host."""
+    import ctypes
+    p = fko.gpp_random(4, 2, 64, 5, seed=1)
+    kw = dict(semantics=fk.FK_GPP_GPPSUM)
+    p64 = fko.GppProblem(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, *[getattr(p, k).astype(np.complex64) for k in
+                                                                  ("aqsm", "aqsn", "I_eps", "wtilde")],
+                         p.vcoul, p.inv_igp_index, p.indinv)
+    with pytest.raises(fk.FkError) as e:
+        fk.gpp_problem(p64, precision="mixed", **kw)
+    assert e.value.code == fk.FK_ERR_UNSUPPORTED and "gppsum" in fk.last_error()
+    for extra, code, word in ((dict(mask=False), fk.FK_ERR_UNSUPPORTED, "gppsum"),
+                              (dict(config=2), fk.FK_ERR_UNSUPPORTED, "gppsum"),
+                              (dict(acht_n1=True), fk.FK_ERR_UNSUPPORTED, "acht_n1_loc"),
+                              (dict(achstemp=True), fk.FK_ERR_UNSUPPORTED, "achstemp"),
+                              (dict(semantics=7), fk.FK_ERR_ARG, "semantics")):
+        with pytest.raises(fk.FkError) as e:
+            fk.gpp_problem(p, **dict(kw, **extra))
+        assert e.value.code == code and word in fk.last_error(), extra
+    wx = np.ascontiguousarray(p.wx)
+    s = fk.GppProblem(p.nb, p.nv, p.ncouls, p.ngpown, len(wx), wx.ctypes.data, 1, 1, 1, 1, 1,
+                      p.inv_igp_index.ctypes.data, p.indinv.ctypes.data, 1e-6, 0.25, 2.5e5, 1.0, fk.FK_GPP_GPPSUM)
+    arr = (ctypes.POINTER(fk.GppProblem) * 1)(ctypes.pointer(s))
+    h = ctypes.c_void_p()
+    opt = fk.default_options()
+    opt.memory = fk.FK_MEM_DEVICE
+    assert fk.lib.gpp_batch_create(1, arr, ctypes.byref(opt), ctypes.byref(h)) == fk.FK_ERR_UNSUPPORTED
+    assert "gppsum" in fk.last_error() and not h.value
+    # the workspace of the form: the canonical layout with 64 x 64 tiles plus one int per column
+    assert fk.lib.gpp_kernel_workspace_size(ctypes.byref(s), ctypes.byref(opt)) > 0
+
+
+def test_python_wrappers_check_buffers_before_the_library(fk, fko):
+    """Wrong dtype, too few elements, a non-contiguous array, a CPU tensor,
+    numpy mixed with torch, or an unknown precision raise in Python before
+    any pointer reaches the library."""
+    import torch
+    p = fko.gpp_random(4, 2, 64, 5, seed=1)
+    args = lambda **o: [o.get(k, getattr(p, k)) for k in ("aqsm", "aqsn", "I_eps", "wtilde", "vcoul")]  # noqa: E731
+    call = lambda *a, **kw: fk.gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, *a, p.inv_igp_index,  # noqa: E731
+                                          p.indinv, **kw)
+    cases = [
+        (TypeError, args(aqsm=p.aqsm.astype(np.complex64)), {}),            # complex64 with fp64
+        (TypeError, args(), dict(precision="mixed")),                         # complex128 with mixed
+        (ValueError, args(), dict(precision="fp32")),                         # unknown precision
+        (ValueError, args(I_eps=p.I_eps[:3]), {}),                            # too few elements
+        (ValueError, args(aqsn=np.asfortranarray(p.aqsn)), {}),               # not C-contiguous
+        (TypeError, args(vcoul=p.vcoul.astype(np.float32)), {}),
+        (TypeError, args(wtilde=torch.from_numpy(p.wtilde)), {}),             # torch mixed with numpy
+        (TypeError, [torch.from_numpy(np.ascontiguousarray(a)) for a in args()], {}),   # CPU tensors
+    ]
+    for exc, a, kw in cases:
+        with pytest.raises(exc):
+            call(*a, **kw)
+    with pytest.raises(ValueError):
+        fk.gpp_kernel(p.nb, p.nv, p.ncouls, p.ngpown, p.wx, *args(), p.inv_igp_index[:2], p.indinv)
+    q = fko.ff_random(4, 32, 8, 2, seed=1)
+    with pytest.raises(ValueError):
+        fk.ff_kernel(q.nb, q.ncouls, q.ngpown, q.nfreq + 1, q.aqsm, q.aqsn, q.I_eps, q.vcoul, q.inv_igp_index,
+                     q.indinv)
+    with pytest.raises(TypeError):
+        fk.ff_kernel(q.nb, q.ncouls, q.ngpown, q.nfreq, q.aqsm, q.aqsn, q.I_eps.astype(np.complex64), q.vcoul,
+                     q.inv_igp_index, q.indinv)
+    with pytest.raises(TypeError):
+        fk.GppBatch([dict(nb=p.nb, nv=p.nv, ncouls=p.ncouls, ngpown=p.ngpown, wx=p.wx, aqsm=p.aqsm, aqsn=p.aqsn,
+                          I_eps=p.I_eps, wtilde=p.wtilde, vcoul=p.vcoul, inv_igp_index=p.inv_igp_index,
+                          indinv=p.indinv)])
+    with pytest.raises(TypeError):
+        fk.fill_real(np.zeros(4), 1, 1, 0.0, 1.0)

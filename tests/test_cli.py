@@ -26,12 +26,21 @@ def _complex_lines(text, name):
     return [out[i] for i in sorted(out)]
 
 
-@pytest.mark.parametrize("args", [[], ["8", "2", "1024"], ["8", "2", "1024", "20", "5"], ["1", "1", "1", "1",
-                                                                                         "--bogus"]])
+@pytest.mark.parametrize("args", [[], ["8", "2", "1024"], ["8", "2", "1024", "20", "1", "5"],
+                                  ["1", "1", "1", "1", "--bogus"]])
 def test_gppker_usage_exits_zero(args):
     r = _run("gppKer", *args)
     assert r.returncode == 0
     assert "The correct form of input is" in r.stdout
+
+
+@pytest.mark.parametrize("args", [["8", "2", "1024", "20", "0"], ["8", "2", "1024", "20", "2"],
+                                  ["8", "2", "1024", "20", "1", "--mixed"]])
+def test_gppker_five_number_form_needs_gppsum_one(args):
+    """The five-number form is gppKerSeq.cpp's command line; only gppsum = 1
+    (FP64) is built, anything else stops before any device work."""
+    r = _run("gppKer", *args)
+    assert r.returncode == 2 and "gppsum = 1" in r.stderr
 
 
 def test_ffker_usage_and_ngpown_check():
@@ -53,6 +62,21 @@ def test_gppker_matches_reference_program():
         for g, w in zip(ach, ref):
             assert abs(g - w) <= 1e-13 * abs(w), (g, w)
         assert "********** Chrono Time Taken **********= " in r.stdout
+
+
+@pytest.mark.gpu
+def test_gppker_gppsum_matches_reference_program():
+    """gppKer <nb> <nv> <ncouls> <npg> 1 prints what the reference's
+    gppKerSeq program prints for gppsum = 1 (its constant inputs)."""
+    d = np.load(os.path.join(GOLDEN, "gppsum_main.npz"))
+    for (nb, nv, C, npg), ref in zip(d["shapes"], d["achtemp"]):
+        print(f"gppKer {nb} {nv} {C} {npg} 1", flush=True)
+        r = _run("gppKer", nb, nv, C, npg, 1)
+        assert r.returncode == 0, r.stderr
+        ach = _complex_lines(r.stdout, "achtemp")
+        assert len(ach) == 3
+        for g, w in zip(ach, ref):
+            assert abs(g - w) <= 1e-13 * abs(w), (g, w)
 
 
 @pytest.mark.gpu
