@@ -465,31 +465,75 @@ def run_ff_batched(fk, torch, steps):
     return res
 
 
-def cpu_baseline(bands, threads):
-    """The reference's own gppKer (oracle/_ref) on `bands` bands of the
-    512/2/32768/20 shape, OMP threads = `threads`."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "gppKer")
-    C, G = NCOULS, NCOULS // NPG
+def _host_cpu():
+    """Model name, ISA flags and core counts of this host."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}, flags
+
+
+def _reference_binary(flags):
+    """The reference's gppKer built for the highest vector ISA level this
+    host runs (AVX-512, else AVX2 + FMA, else generic x86-64)."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    for name, need in (("gppKer_x86v4", "avx512f"), ("gppKer_x86v3", "avx2"), ("gppKer", None)):
+        exe = os.path.join(ref, name)
+        if os.access(exe, os.X_OK) and (need is None or need in flags):
+            return exe, name
+    return None, None
+
+
+def cpu_baseline(bands, threads, runs=3):
+    """The reference's own OpenMP gppKer program (BGW-Kernels/gppKer.cpp,
+    built from its sources into oracle/_ref) on this host's cores: the
+    median of `runs` runs of the full 128/2/8192/20 problem and of a
+    `bands`-band sample of 512/2/32768/20 (the metric's shape); the timed
+    region is the program's own band loop (its chrono line)."""
+    host, flags = _host_cpu()
+    exe, name = _reference_binary(flags)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores",
                OMP_STACKSIZE="16M")
-    if os.access(exe, os.X_OK):
-        log(f"cpu baseline: reference gppKer {bands} {NV} {C} {NPG} on {threads} threads")
-        out = subprocess.run([exe, str(bands), str(NV), str(C), str(NPG)], env=env, capture_output=True,
-                             text=True, timeout=600, check=True).stdout
-        secs = float(re.search(r"Chrono Time Taken \*+= ([0-9.eE+-]+)", out).group(1))
-        kind = "reference"
-    else:
-        from oracle import fko
-        log(f"cpu baseline: oracle port, {bands} bands on {threads} threads")
-        p = fko.gpp_random(bands, NV, C, G, seed=1)
-        t0 = time.perf_counter()
-        fko.gpp(p, extended=False if threads == 1 else True, nthreads=threads)
-        secs = time.perf_counter() - t0
-        kind = "port"
-    U = bands * G * C * NW
-    return {"value": U / secs, "unit": "updates/s", "cores": threads, "kind": kind,
-            "sample": f"{bands} of {NB} bands, nvband {NV}, ncouls {C}, ngpown {G}, 3 frequencies "
-                      f"({U:.3e} updates, {secs:.2f} s)"}
+    cases = [("128/2/8192/20, all 128 bands", 128, NV, 8192, NPG),
+             (f"{bands} of the 512 bands of 512/2/32768/20", bands, NV, NCOULS, NPG)]
+    out = {}
+    for label, nb, nv, C, npg in cases:
+        G = C // npg
+        U = nb * G * C * NW
+        secs = []
+        for k in range(runs):
+            if exe:
+                log(f"cpu baseline: {name} {nb} {nv} {C} {npg} on {threads} threads (run {k + 1}/{runs})")
+                text = subprocess.run([exe, str(nb), str(nv), str(C), str(npg)], env=env, capture_output=True,
+                                      text=True, timeout=600, check=True).stdout
+                secs.append(float(re.search(r"Chrono Time Taken \*+= ([0-9.eE+-]+)", text).group(1)))
+            else:
+                from oracle import fko
+                log(f"cpu baseline: oracle port {nb}/{nv}/{C}/{G} on {threads} threads (run {k + 1}/{runs})")
+                p = fko.gpp_random(nb, nv, C, G, seed=1)
+                t0 = time.perf_counter()
+                fko.gpp(p, extended=False if threads == 1 else True, nthreads=threads)
+                secs.append(time.perf_counter() - t0)
+        med = float(np.median(secs))
+        out[label] = {"updates": U, "seconds": secs, "median_s": med, "updates_per_s": U / med}
+    sample = out[cases[1][0]]
+    return {"value": sample["updates_per_s"], "unit": "updates/s", "cores": threads,
+            "kind": "reference" if exe else "port",
+            "sample": f"{cases[1][0]}, nvband {NV}, 3 frequencies: median of {runs} runs "
+                      f"({sample['updates']:.3e} updates, {sample['median_s']:.2f} s)",
+            "binary": name or "oracle port (fko.gpp)",
+            "build": "g++ -O3 -std=c++11 -fopenmp -march=" + {"gppKer_x86v4": "x86-64-v4", "gppKer_x86v3": "x86-64-v3",
+                                                               "gppKer": "x86-64"}.get(name, "n/a"),
+            "omp": "OMP_PROC_BIND=spread OMP_PLACES=cores OMP_STACKSIZE=16M",
+            "host": host, "configs": out}
 
 
 def load_traffic(name):
@@ -512,8 +556,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-singles", action="store_true",
                     help="also time each batched FP64 problem alone (gpp_kernel, best configuration)")
-    ap.add_argument("--cpu-bands", type=int, default=256)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-bands", type=int, default=128)
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0))),
+                    help="host threads for the CPU baseline (default: OMP_NUM_THREADS, the host's CPU share "
+                         "for this GPU, else every core this process may run on)")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -586,7 +634,7 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: 3 x {args.calibrate_copy} MiB")
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_bands, args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_bands, args.cpu_threads, args.cpu_runs)
     print(json.dumps(line), flush=True)
 
 
