@@ -54,6 +54,13 @@ def gpp_alg_flops(nb, nv, C, G, nw):
     return 8.0 * nb * G * C + G * C * (8.0 * nv + 9.0 + 42.0 * nw)
 
 
+def gpp_executed_flops(nb, nv, C, G, nw):
+    """FP64 flops the kernel executes for the same work: the band product
+    runs as three real products per complex multiply-add (the 3M method),
+    6 nb G C, instead of the 8 nb G C that gpp_alg_flops counts."""
+    return 6.0 * nb * G * C + G * C * (8.0 * nv + 9.0 + 42.0 * nw)
+
+
 def gpp_reference_flops(nb, nv, C, G, nw):
     """The per-update count of the unfactorised loop with band-invariant work
     hoisted: 8U + 6 nb G C + 22 G C nw + 10 nv G C nw."""
@@ -204,7 +211,11 @@ def run_ff(fk, torch, steps, warmup, seed=4):
            "roofline": {"bound": "mfma", "kernel": "ff_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                         "traffic": load_traffic("ff_fused_fp64_traffic.json"),
-                        "alg_flops_per_launch": flops, "kernel_ms": kernel_ms}}
+                        "alg_flops_per_launch": flops, "kernel_ms": kernel_ms,
+                        "executed_flops_per_launch": 0.75 * flops,
+                        "executed_frac": 0.75 * flops / (kernel_ms * 1e-3) / 1e12 / PEAK_FP64,
+                        "note": "alg counts a complex multiply-add as 8 flops; the kernel runs 3 real MFMA "
+                                "products per complex product (the 3M method), 3/4 of that"}}
 
     # the same contraction recomputed in split BF16 (three slices, six slice
     # products on MFMA, FP32 accumulation per ig chunk, FP64 refinement)
@@ -578,6 +589,7 @@ def main():
 
     r = run_gpp(fk, torch, "fp64", args.steps, args.warmup)
     f_alg = gpp_alg_flops(NB, NV, C, G, NW)
+    f_exec = gpp_executed_flops(NB, NV, C, G, NW)
     achieved = f_alg / (r["kernel_ms"] * 1e-3) / 1e12
     line = {
         "metric": METRIC,
@@ -597,7 +609,12 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "gpp_fused_kernel", "achieved": achieved, "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": load_traffic("gpp_fused_fp64_traffic.json"),
-                     "alg_flops_per_launch": f_alg, "kernel_ms": r["kernel_ms"]},
+                     "alg_flops_per_launch": f_alg, "kernel_ms": r["kernel_ms"],
+                     "executed_flops_per_launch": f_exec,
+                     "executed_tflops": f_exec / (r["kernel_ms"] * 1e-3) / 1e12,
+                     "executed_frac": f_exec / (r["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64,
+                     "note": "alg counts a complex multiply-add as 8 flops; the kernel runs the band product "
+                             "as 3 real MFMA products (6 flops), so executed = the MFMA and VALU work done"},
         "reference_equivalent_tflops": gpp_reference_flops(NB, NV, C, G, NW) / (r["ms_per_step"] * 1e-3) / 1e12,
         "event_ms_per_step": r["event_ms_per_step"],
     }

@@ -52,7 +52,7 @@ struct GppArgs {
     int nb, nv, C, G, nw, ldc, Kpad;
     int nvpad;             // occupied bands rounded up to a k-step (4)
     int nks;               // k-steps (Kpad / 4)
-    int tiles_m, tiles_n;
+    int tiles_m, tiles_n, group_m;
     int stagger_lo, stagger_hi;   // workgroups that start late ...
     int stagger_ticks;            // ... by this many wall-clock ticks (100 MHz)
     PoleParams pp;
@@ -132,14 +132,23 @@ template <class E> __device__ __forceinline__ E zero_elem();
 template <> __device__ __forceinline__ double2 zero_elem<double2>() { return make_double2(0.0, 0.0); }
 template <> __device__ __forceinline__ float2 zero_elem<float2>() { return make_float2(0.f, 0.f); }
 
-// Workgroup b -> logical tile, so that the tiles one XCD runs back to back
-// share an aqsn column block (tm varies fastest within an XCD's range).
-__device__ __forceinline__ void tile_coords(int b, int tiles_m, int total, int &tm, int &tn)
+// Workgroup b -> logical tile.  Each XCD (every kXcds-th workgroup) runs a
+// contiguous range of the logical order, and the logical order walks bands
+// of group_m row tiles column by column: the tiles an XCD runs together
+// form a block of group_m row tiles x (its workgroups / group_m) column
+// tiles, so both the coefficient panels and the aqsn panels they read are
+// shared in that XCD's L2.  group_m = tiles_m walks whole columns.
+__device__ __forceinline__ void tile_coords(int b, int tiles_m, int tiles_n, int group_m, int &tm, int &tn)
 {
-    int q = total / kXcds, r = total % kXcds, x = b % kXcds, i = b / kXcds;
-    int t = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-    tm = t % tiles_m;
-    tn = t / tiles_m;
+    const int total = tiles_m * tiles_n;
+    const int q = total / kXcds, r = total % kXcds, x = b % kXcds, i = b / kXcds;
+    const int t = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    const int band = t / (group_m * tiles_n);
+    const int first = band * group_m;
+    const int gm = min(tiles_m - first, group_m);
+    const int u = t - band * group_m * tiles_n;
+    tm = first + u % gm;
+    tn = u / gm;
 }
 
 
@@ -244,14 +253,29 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
     const int nmax = a.C - 1;
     const int kl = lane >> 4, cl = lane & 15;
 
-    A4 acc_re[MB][NB], acc_im[MB][NB];
+    // FP64: the complex product as three real GEMMs (the 3M method; Higham,
+    // SIAM J. Matrix Anal. Appl. 13(3), 1992):
+    //   T1 = (cr + ci) ar,  T2 = cr (ai - ar),  T3 = ci (ar + ai),
+    //   re B = T1 - T3,     im B = T1 + T2
+    // (acc_re, acc_im, acc_t3 hold T1, T2, T3), 12 MFMAs per k-step where
+    // four real products take 16.  FP32 (the mixed tiles) keeps the four.
+    constexpr bool K3 = sizeof(E) == sizeof(double2);
+    A4 acc_re[MB][NB], acc_im[MB][NB], acc_t3[K3 ? MB : 1][K3 ? NB : 1];
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             acc_re[i][j] = A4{0, 0, 0, 0};
             acc_im[i][j] = A4{0, 0, 0, 0};
+            if constexpr (K3) acc_t3[i][j] = A4{0, 0, 0, 0};
         }
+    // B of accumulator register r of block (i, j)
+    auto acc_value = [&](int i, int j, int r) -> double2 {
+        if constexpr (K3)
+            return make_double2(acc_re[i][j][r] - acc_t3[i][j][r], acc_re[i][j][r] + acc_im[i][j][r]);
+        else
+            return make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
+    };
     E *occ = &occ_lds[wave][0];
     auto occ_store = [&](bool zero) {
 #pragma unroll
@@ -260,9 +284,10 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
             for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
+                    const double2 b = acc_value(i, j, r);
                     E v;
-                    v.x = zero ? 0 : acc_re[i][j][r];
-                    v.y = zero ? 0 : acc_im[i][j][r];
+                    v.x = zero ? 0 : b.x;
+                    v.y = zero ? 0 : b.y;
                     occ[((i * NB + j) * 4 + r) * 64 + lane] = v;
                 }
     };
@@ -289,20 +314,39 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
         for (int j = 0; j < NB; ++j) f.b[j] = gld<TABLE>(qb, colj[j]);
     };
     auto mma = [&](const Frags<E, MB, NB> &f) {
+        if constexpr (K3) {
+            double as[MB], bd[NB], bs[NB];
 #pragma unroll
-        for (int i = 0; i < MB; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                acc_re[i][j] = mfma(f.a[i].x, f.b[j].x, acc_re[i][j]);
-                acc_im[i][j] = mfma(f.a[i].x, f.b[j].y, acc_im[i][j]);
-            }
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
+            for (int i = 0; i < MB; ++i) as[i] = f.a[i].x + f.a[i].y;
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                acc_re[i][j] = mfma_nega(f.a[i].y, f.b[j].y, acc_re[i][j]);
-                acc_im[i][j] = mfma(f.a[i].y, f.b[j].x, acc_im[i][j]);
+                bd[j] = f.b[j].y - f.b[j].x;
+                bs[j] = f.b[j].x + f.b[j].y;
             }
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    acc_re[i][j] = mfma(as[i], f.b[j].x, acc_re[i][j]);
+                    acc_im[i][j] = mfma(f.a[i].x, bd[j], acc_im[i][j]);
+                    acc_t3[i][j] = mfma(f.a[i].y, bs[j], acc_t3[i][j]);
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    acc_re[i][j] = mfma(f.a[i].x, f.b[j].x, acc_re[i][j]);
+                    acc_im[i][j] = mfma(f.a[i].x, f.b[j].y, acc_im[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < MB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    acc_re[i][j] = mfma_nega(f.a[i].y, f.b[j].y, acc_re[i][j]);
+                    acc_im[i][j] = mfma(f.a[i].y, f.b[j].x, acc_im[i][j]);
+                }
+        }
     };
 
     // k-steps of the occupied bands come first; after the last of them the
@@ -360,7 +404,7 @@ __device__ __forceinline__ void gpp_tile(const GppArgs<E> &a, int tm, int tn, Ti
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const bool ok = ((rowok >> (i * 4 + r)) & 1u) && col <= nmax;
-                    const double2 B = make_double2((double)acc_re[i][j][r], (double)acc_im[i][j][r]);
+                    const double2 B = acc_value(i, j, r);
                     const double2 Bo = widen(occ[((i * NB + j) * 4 + r) * 64 + lane]);
                     pole_element<NWC>(a.pp, w0, ok, widen(cwt[r]), widen(cie[r]), B, Bo, want_ssx, sach, sasx);
                     if constexpr (ACHS) {
@@ -425,7 +469,7 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
         while (wall_clock64() - t0 < (uint64_t)a.stagger_ticks) __builtin_amdgcn_s_sleep(32);
     }
     int tm, tn;
-    tile_coords(blockIdx.x, a.tiles_m, a.tiles_m * a.tiles_n, tm, tn);
+    tile_coords(blockIdx.x, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
     gpp_tile<E, MB, NB, P, WGM, WGN, NWC, ACHS>(a, tm, tn, sh);
 }
 
@@ -545,6 +589,12 @@ __global__ void widen_kernel(const float2 *src, int64_t n, double2 *dst)
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+#ifndef FK_GPP_GROUP_M
+#define FK_GPP_GROUP_M 1 << 20
+#endif
+// row tiles per band of the fused kernel's tile order (tile_coords)
+constexpr int kGroupM = FK_GPP_GROUP_M;
+
 struct TileCfg { int bm, bn, p, minw; };
 // config ids (fk_options.config = id + 1): wave tile, ring depth, waves, WG/CU
 //   0: 32x64 waves, ring 4, 1x4 waves (32x256 tile), 1 workgroup/CU
@@ -716,6 +766,7 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     a.nb = p->number_bands; a.nv = p->nvband; a.C = C; a.G = G; a.nw = p->nfreq; a.ldc = L.ldc; a.Kpad = L.Kpad;
     a.nvpad = L.nvpad; a.nks = L.Kpad / 4;
     a.tiles_m = L.tiles_m; a.tiles_n = L.tiles_n;
+    a.group_m = std::min(L.tiles_m, kGroupM);
     const int tiles = L.tiles_m * L.tiles_n;
     a.stagger_lo = a.stagger_hi = 0;
     a.stagger_ticks = 0;
@@ -1014,6 +1065,7 @@ struct GppBatch {
             a.ldc = b.ldc; a.Kpad = b.Kpad; a.nvpad = b.nvpad; a.nks = b.Kpad / 4;
             a.tiles_m = b.ldc / t.bm;
             a.tiles_n = div_up(b.C, t.bn);
+            a.group_m = a.tiles_m;   // unused: the batch walks its own tile table
             a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
             a.pp = make_pole_params(p.wx, p.nfreq, p.limittwo, p.limitone, p.to1, opt.mask != 0);
             if (b.nb > 0) order.push_back({{-a.nks, -b.nw}, i});
