@@ -33,7 +33,7 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "ff_kernel_workspace_size", "fk_last_error", "fk_abi_version", "fk_build_target",
            "fk_fill_complex", "fk_fill_real", "fk_complex_apply", "fk_mfma_probe",
            "gpp_batch_create", "gpp_batch_run", "gpp_batch_config", "gpp_batch_workspace_bytes",
-           "gpp_batch_destroy", "gpp_batch_branches", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size",
+           "gpp_batch_destroy", "gpp_kernel_batched", "ff_kernel_full", "ff_kernel_full_workspace_size",
            "gpp_kernel_ex", "fk_gpp_instance_write", "fk_ff_instance_write", "fk_instance_open",
            "fk_instance_kind", "fk_instance_complex_bytes", "fk_instance_gpp", "fk_instance_ff", "fk_instance_close",
            "fk_instance_list_read", "fk_instance_list_free", "ff_batch_create", "ff_batch_run", "ff_batch_config",
@@ -185,8 +185,6 @@ def _load():
     L.gpp_batch_run.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), P, ctypes.POINTER(ctypes.c_float)]
     L.gpp_batch_config.restype = ctypes.c_int
     L.gpp_batch_config.argtypes = [P]
-    L.gpp_batch_branches.restype = ctypes.c_int
-    L.gpp_batch_branches.argtypes = [P]
     L.gpp_batch_workspace_bytes.restype = SZ
     L.gpp_batch_workspace_bytes.argtypes = [P]
     L.gpp_batch_destroy.restype = None
@@ -474,8 +472,10 @@ class GppBatch:
     problems: sequence of dicts with the gpp_kernel arguments: nb, nv, ncouls,
     ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul (torch device tensors),
     inv_igp_index, indinv and optionally to1, limittwo, limitone, occ.
-    config 0 lets the library time every tile configuration on the batch.
-    Mixed (complex64 tensors) runs the tile kernel with FP32 MFMA."""
+    FP64: config 0 lets the library time every tile configuration on the
+    batch.  Mixed (complex64 tensors): config 0 runs the int8-split
+    contraction of gpp_kernel's mixed path, config 1..4 the tile kernel with
+    FP32 MFMA."""
 
     def __init__(self, problems, precision="fp64", config=0, mask=True):
         self._keep = []
@@ -516,11 +516,6 @@ class GppBatch:
     @property
     def config(self):
         return lib.gpp_batch_config(self._h)
-
-    @property
-    def branches(self):
-        """Branches of the int8-split graph (0 for the tile kernel)."""
-        return lib.gpp_batch_branches(self._h)
 
     @property
     def workspace_bytes(self):

@@ -1237,30 +1237,23 @@ struct GppBatch {
     }
 };
 
-// Mixed-precision batches on the int8-split contraction -- switched off
-// (kOzBatchEnabled): its first GPU run ended in a host segmentation fault
-// inside gpp_batch_create whose cause is not found (a stub-runtime run of the
-// same host code is clean under AddressSanitizer), so batches keep the FP32
-// tile kernel until it is.  Every problem runs
-// the kernels of the single-problem path (split, int8 GEMM, pole epilogue,
-// reduction) on one of K branches of one captured graph: a branch takes its
-// problems in turn and reuses one workspace, sized to its largest problem,
-// while the branches overlap on the GPU.  Problems are dealt largest first
-// to the branch with the least work so far (nb * ngpown * ncouls).  K comes
-// from the caller or from timing K = 1, 2, 4, 8 on the batch itself.
-constexpr bool kOzBatchEnabled = false;
-
+// Mixed-precision batches on the int8-split contraction: the kernels of the
+// single-problem mixed path (operand slices, int8 GEMM, pole epilogue) as
+// one launch each for all problems (fk_gpp_oz.hip, gpp_oz_batch_launch),
+// then the batched reduction; the launches of one run are captured into a
+// graph on one stream, as for the FP64 batch.  Every problem keeps its own
+// workspace (its operand slices and FP64 contraction B, [ldm][ldn]) in one
+// allocation.  (An earlier form ran every problem's launches on parallel
+// branches of a multi-stream capture; it ended in a host crash inside
+// gpp_batch_create on its first run, with the cause not found, and was
+// removed.)
 struct OzBatch {
-    static constexpr int kMaxBranches = 8;
-    int n = 0, nbranch = 0;
+    int n = 0;
     std::vector<fk_gpp_problem> src;
-    std::vector<size_t> need;             // workspace bytes of each problem
     void *mem = nullptr;
-    size_t bytes = 0, off_out = 0, off_nw = 0, off_ptrs = 0, off_branches = 0;
-    std::vector<size_t> off_igp;
+    size_t bytes = 0, off_rec = 0, off_first = 0, off_red = 0, off_out = 0, off_nw = 0, off_ptrs = 0;
+    int total_prep = 0, total_gemm = 0, total_pole = 0;
     hipStream_t cs = nullptr;
-    hipStream_t side[kMaxBranches - 1] = {};
-    hipEvent_t ev[kMaxBranches] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     fk_options opt;
@@ -1269,43 +1262,27 @@ struct OzBatch {
 
     ~OzBatch()
     {
-        drop_graph();
-        if (cs) (void)hipStreamSynchronize(cs);
-        for (hipStream_t &x : side)
-            if (x) { (void)hipStreamSynchronize(x); (void)hipStreamDestroy(x); }
-        for (hipEvent_t &e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (cs) (void)hipStreamDestroy(cs);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cs) {
+            (void)hipStreamSynchronize(cs);
+            (void)hipStreamDestroy(cs);
+        }
         if (mem) (void)hipFree(mem);
     }
 
-    void drop_graph()
+    int enqueue(hipStream_t s)
     {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        const int *first = reinterpret_cast<const int *>(at(off_first));
+        FK_TRY(gpp_oz_batch_launch(at(off_rec), first, first + (n + 1), first + 2 * (n + 1), n, total_prep, total_gemm,
+                                   total_pole, s));
+        hipLaunchKernelGGL(gpp_reduce_batch_kernel, dim3(2 * kMaxFreq, n), dim3(256), 0, s,
+                           reinterpret_cast<const ReduceBatch *>(at(off_red)), reinterpret_cast<double2 *>(at(off_out)));
+        FK_HIP(hipGetLastError());
+        return FK_OK;
     }
 
-    // problem -> branch for K branches, and each branch's workspace bytes
-    void assign(int K, std::vector<int> &branch_of, std::vector<size_t> &size) const
-    {
-        std::vector<std::pair<double, int>> order;
-        for (int i = 0; i < n; ++i)
-            order.push_back({-(double)src[i].number_bands * src[i].ngpown * src[i].ncouls, i});
-        std::stable_sort(order.begin(), order.end());
-        std::vector<double> load(K, 0.0);
-        branch_of.assign(n, 0);
-        size.assign(K, 0);
-        for (const auto &o : order) {
-            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            branch_of[o.second] = b;
-            load[b] -= o.first;
-            size[b] = std::max(size[b], align_up(need[o.second], 256));
-        }
-    }
-
-    int create(int n_, const fk_gpp_problem *const *problems, const fk_options *o, int branches)
+    int create(int n_, const fk_gpp_problem *const *problems, const fk_options *o)
     {
         n = n_;
         opt = *o;
@@ -1314,120 +1291,73 @@ struct OzBatch {
         opt.workspace_bytes = 0;
         for (int i = 0; i < n; ++i) src.push_back(*problems[i]);
         if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+            cs = nullptr;
             set_error("gpp_batch_create: stream creation failed");
             return FK_ERR_HIP;
         }
-        for (hipStream_t &x : side) FK_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-        for (hipEvent_t &e : ev) FK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const size_t rec_bytes = gpp_oz_batch_record_bytes();
         Scratch plan;
+        off_rec = plan.plan(rec_bytes * n);
+        off_first = plan.plan(sizeof(int) * 3 * (size_t)(n + 1));
+        off_red = plan.plan(sizeof(ReduceBatch) * n);
         off_out = plan.plan(sizeof(double2) * 2 * kMaxFreq * n);
         off_nw = plan.plan(sizeof(int) * n);
         off_ptrs = plan.plan(sizeof(void *) * 2 * n);
+        std::vector<size_t> off_igp(n), off_ws(n), ws_bytes(n);
         for (int i = 0; i < n; ++i) {
-            off_igp.push_back(plan.plan(sizeof(int) * (size_t)src[i].ngpown));
+            off_igp[i] = plan.plan(sizeof(int) * (size_t)src[i].ngpown);
             Scratch ws;
             OzLayout L;
             gpp_oz_plan(&src[i], ws, L);
-            need.push_back(src[i].number_bands > 0 ? ws.total() : 0);
+            ws_bytes[i] = ws.total();       // also without bands: every record points into the allocation
+            off_ws[i] = plan.plan(ws_bytes[i]);
         }
-        off_branches = plan.total();
-        const std::vector<int> cand = branches > 0 ? std::vector<int>{branches} : std::vector<int>{1, 2, 4, 8};
-        size_t most = 0;
-        for (int K : cand) {
-            std::vector<int> bo;
-            std::vector<size_t> sz;
-            assign(K, bo, sz);
-            size_t t = 0;
-            for (size_t x : sz) t += x;
-            most = std::max(most, t);
-        }
-        bytes = off_branches + most;
+        bytes = plan.total();
         if (hipMalloc(&mem, bytes) != hipSuccess) {
             mem = nullptr;
             set_error("gpp_batch_create: hipMalloc(%zu) failed", bytes);
             return FK_ERR_NOMEM;
         }
-        std::vector<int> nw(n);
+        std::vector<char> recs(rec_bytes * n);
+        std::vector<int> first(3 * (size_t)(n + 1), 0), nw(n);
+        std::vector<ReduceBatch> red(n);
         for (int i = 0; i < n; ++i) {
+            const fk_gpp_problem &p = src[i];
             std::vector<int32_t> cols;
-            FK_TRY(resolve_columns(src[i].ncouls, src[i].ngpown, src[i].inv_igp_index, src[i].indinv, cols));
+            FK_TRY(resolve_columns(p.ncouls, p.ngpown, p.inv_igp_index, p.indinv, cols));
             FK_HIP(hipMemcpyAsync(at(off_igp[i]), cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice, cs));
             FK_HIP(hipStreamSynchronize(cs));   // cols leaves scope
-            nw[i] = src[i].nfreq;
-        }
-        FK_HIP(hipMemcpyAsync(at(off_nw), nw.data(), sizeof(int) * n, hipMemcpyHostToDevice, cs));
-        FK_HIP(hipStreamSynchronize(cs));
-        if (cand.size() == 1) return capture(cand[0]);
-        hipEvent_t e0, e1;
-        FK_HIP(hipEventCreate(&e0));
-        FK_HIP(hipEventCreate(&e1));
-        int best = cand[0], rc = FK_OK;
-        float best_ms = 1e30f;
-        for (int K : cand) {
-            rc = capture(K);
-            if (rc != FK_OK) break;
-            float ms = 0.f;
-            if (hipGraphLaunch(exec, cs) != hipSuccess || hipEventRecord(e0, cs) != hipSuccess ||
-                hipGraphLaunch(exec, cs) != hipSuccess || hipEventRecord(e1, cs) != hipSuccess ||
-                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                set_error("gpp_batch_create: timing the branch counts failed");
-                rc = FK_ERR_HIP;
-                break;
-            }
-            if (ms < best_ms) { best_ms = ms; best = K; }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc != FK_OK) return rc;
-        return nbranch == best ? FK_OK : capture(best);
-    }
-
-    // one run as a graph on K branches
-    int capture(int K)
-    {
-        drop_graph();
-        std::vector<int> branch_of;
-        std::vector<size_t> size;
-        assign(K, branch_of, size);
-        std::vector<size_t> base(K, off_branches);
-        for (int b = 1; b < K; ++b) base[b] = base[b - 1] + size[b - 1];
-        FK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        int rc = FK_OK;
-        auto hip = [&](hipError_t e) { if (rc == FK_OK && e != hipSuccess) { set_error("gpp_batch_create: %s", hipGetErrorString(e)); rc = FK_ERR_HIP; } };
-        hip(hipEventRecord(ev[0], cs));
-        for (int b = 1; b < K; ++b) hip(hipStreamWaitEvent(side[b - 1], ev[0], 0));
-        double2 *out = reinterpret_cast<double2 *>(at(off_out));
-        for (int i = 0; i < n && rc == FK_OK; ++i) {
-            const int b = branch_of[i];
-            hipStream_t s = b == 0 ? cs : side[b - 1];
-            const fk_gpp_problem &p = src[i];
-            double2 *da = out + (size_t)2 * i * kMaxFreq, *dx = da + kMaxFreq;
-            if (p.number_bands == 0) {
-                hipLaunchKernelGGL(zero_outputs_kernel, dim3(1), dim3(64), 0, s, da, dx, p.nfreq, nullptr);
-                hip(hipGetLastError());
-                continue;
-            }
             Scratch ws;
             OzLayout L;
             gpp_oz_plan(&p, ws, L);
-            rc = ws.acquire(at(base[b]), size[b], s);
-            if (rc == FK_OK)
-                rc = gpp_oz_launch(&p, &opt, L, ws, reinterpret_cast<const int *>(at(off_igp[i])),
-                                   static_cast<const float2 *>(p.aqsmtemp), static_cast<const float2 *>(p.aqsntemp),
-                                   static_cast<const float2 *>(p.I_eps_array),
-                                   static_cast<const float2 *>(p.wtilde_array), p.vcoul, da, dx, nullptr, s);
+            FK_TRY(ws.acquire(at(off_ws[i]), ws_bytes[i], cs));
+            const OzBlocks nbk = gpp_oz_batch_record(&p, &opt, L, ws, reinterpret_cast<const int *>(at(off_igp[i])),
+                                                     recs.data() + rec_bytes * i);
+            first[i + 1] = first[i] + nbk.prep;
+            first[(n + 1) + i + 1] = first[(n + 1) + i] + nbk.gemm;
+            first[2 * (n + 1) + i + 1] = first[2 * (n + 1) + i] + nbk.pole;
+            red[i].partials = static_cast<const double2 *>(ws.at(L.off_part));
+            red[i].tiles = nbk.pole;
+            red[i].nw = p.nfreq;
+            red[i].asx_scale = 2.0 * p.occ;
+            nw[i] = p.nfreq;
         }
-        for (int b = 1; b < K; ++b) {
-            hip(hipEventRecord(ev[b], side[b - 1]));
-            hip(hipStreamWaitEvent(cs, ev[b], 0));
-        }
+        total_prep = first[n];
+        total_gemm = first[(n + 1) + n];
+        total_pole = first[2 * (n + 1) + n];
+        FK_HIP(hipMemcpyAsync(at(off_rec), recs.data(), recs.size(), hipMemcpyHostToDevice, cs));
+        FK_HIP(hipMemcpyAsync(at(off_first), first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, cs));
+        FK_HIP(hipMemcpyAsync(at(off_red), red.data(), sizeof(ReduceBatch) * n, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipMemcpyAsync(at(off_nw), nw.data(), sizeof(int) * n, hipMemcpyHostToDevice, cs));
+        FK_HIP(hipStreamSynchronize(cs));
+        FK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(cs);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(cs, &g);
         graph = g;
         if (rc != FK_OK) return rc;
         if (e != hipSuccess) { set_error("gpp_batch_create: graph capture: %s", hipGetErrorString(e)); return FK_ERR_HIP; }
         FK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        nbranch = K;
         return FK_OK;
     }
 
@@ -1627,9 +1557,7 @@ struct fk_gpp_batch {
 
 namespace {
 
-// oz_branches: branches of the int8-split graph (0 = time 1, 2, 4, 8)
-int create_batch(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options, int oz_branches,
-                 fk_gpp_batch **batch)
+int create_batch(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options, fk_gpp_batch **batch)
 {
     if (!batch) { set_error("gpp_batch_create: NULL result pointer"); return FK_ERR_ARG; }
     *batch = nullptr;
@@ -1661,11 +1589,11 @@ int create_batch(int32_t n, const fk_gpp_problem *const *problems, const fk_opti
         FK_TRY(resolve_columns(problems[i]->ncouls, problems[i]->ngpown, problems[i]->inv_igp_index,
                                problems[i]->indinv, cols));
     }
-    if (kOzBatchEnabled && options->precision == FK_PREC_MIXED && options->config == 0) {
+    if (options->precision == FK_PREC_MIXED && options->config == 0) {
         for (int i = 0; i < n; ++i) FK_TRY(gpp_oz_check(problems[i]));
         fk_gpp_batch *h = new fk_gpp_batch;
         h->oz.reset(new OzBatch);
-        const int rc = h->oz->create(n, problems, options, oz_branches);
+        const int rc = h->oz->create(n, problems, options);
         if (rc != FK_OK) {
             delete h;
             return rc;
@@ -1704,7 +1632,7 @@ int gpp_batch_create(int32_t n, const fk_gpp_problem *const *problems, const fk_
                      fk_gpp_batch **batch)
 {
     clear_error();
-    return create_batch(n, problems, options, 0, batch);
+    return create_batch(n, problems, options, batch);
 }
 
 int gpp_batch_run(fk_gpp_batch *batch, fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, void *stream,
@@ -1725,11 +1653,6 @@ int gpp_batch_config(const fk_gpp_batch *batch)
     return batch->oz ? 0 : batch->b.cfg + 1;
 }
 
-int gpp_batch_branches(const fk_gpp_batch *batch)
-{
-    return batch && batch->oz ? batch->oz->nbranch : 0;
-}
-
 size_t gpp_batch_workspace_bytes(const fk_gpp_batch *batch)
 {
     if (!batch) return 0;
@@ -1746,11 +1669,11 @@ int gpp_kernel_batched(int32_t n, const fk_gpp_problem *const *problems, const f
 {
     clear_error();
     fk_options o = options ? *options : fk_default_options();
-    // one call: no autotuning; the single-problem default tiles, or (mixed,
-    // when the int8-split graph is on) four branches of that graph
-    if (o.config == 0 && (o.precision != FK_PREC_MIXED || !kOzBatchEnabled)) o.config = 2;
+    // one call: no autotuning; the single-problem default tiles, or (mixed)
+    // the int8-split contraction
+    if (o.config == 0 && o.precision != FK_PREC_MIXED) o.config = 2;
     fk_gpp_batch *b = nullptr;
-    FK_TRY(create_batch(n, problems, &o, 4, &b));
+    FK_TRY(create_batch(n, problems, &o, &b));
     hipStream_t s = static_cast<hipStream_t>(o.stream);
     int rc = gpp_batch_run(b, achtemp, asxtemp, s, o.kernel_ms);
     if (rc == FK_OK && hipStreamSynchronize(s) != hipSuccess) {

@@ -40,6 +40,7 @@
 #include "fk_internal.h"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 namespace fk {
@@ -167,10 +168,15 @@ __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
 
 constexpr int kPrepRowsA = 4, kPrepColsB = 16;
 
+__device__ __forceinline__ void oz_prep_block(const PrepArgs &a, int b)
+{
+    if (b < a.blocks_a) prep_slab<kPrepRowsA, true>(a, b);
+    else prep_slab<kPrepColsB, false>(a, b - a.blocks_a);
+}
+
 __global__ __launch_bounds__(256) void oz_prep_kernel(PrepArgs a)
 {
-    if ((int)blockIdx.x < a.blocks_a) prep_slab<kPrepRowsA, true>(a, blockIdx.x);
-    else prep_slab<kPrepColsB, false>(a, blockIdx.x - a.blocks_a);
+    oz_prep_block(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -268,14 +274,15 @@ struct HalfFrags {
     i4 b[kSlices];           // [slice]
 };
 
-__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
+// one 64x64 tile (workgroup b of the problem's launch)
+__device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
-    oz_tile(blockIdx.x, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
+    oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
     const int nks = a.nkh / 2;
 
     // global source of this wave's fragment loads: stage fragment f is A
@@ -397,14 +404,32 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
     }
 }
 
+__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
+{
+    oz_gemm_tile(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Pole/SX epilogue over every (my_igp, ig): B from bmat, Bo = sum over the
 // occupied bands in FP64, then the FP64 pole and SX factors.  Workgroup b
-// takes row my_igp = b / chunks and the b % chunks-th of its columns, so
-// the row's constants (igp, its scale, the occupied coefficients) are read
-// once and the element loads are 32-bit offsets from row bases; one
-// partial per workgroup and frequency, in a fixed order.
+// takes row my_igp = b / chunks and the (b % chunks)-th of its columns, in
+// blocks of kPoleCols, kPoleK columns per thread (tid, tid + 256, ...).  For
+// each block it loads the elements once, forms the per-element products
+// (B I_eps, Bo Omega2, Bo I_eps) once, and keeps them in registers across
+// the frequency passes;
+// a pass takes up to 4 frequencies (nw split into ceil(nw / 4) passes of
+// near-equal width, so no frequency slot is evaluated for nothing).  The
+// occupied coefficients of the row are staged in LDS.  One partial per
+// workgroup and frequency, summed in a fixed order (over the threads per
+// block, then over the blocks).
 // ---------------------------------------------------------------------------
+#ifndef FK_POLE_K
+#define FK_POLE_K 2
+#endif
+constexpr int kPoleK = FK_POLE_K;             // columns per thread
+constexpr int kPoleCols = 256 * kPoleK;       // columns per workgroup
+constexpr int kOccLds = 64;                   // occupied coefficients staged in LDS
+
 struct PoleArgs {
     const double2 *bmat;     // [ldm][ldn] B / (ra[m] rb[ig])
     const double *ra, *rb;   // row and column scales of the slices
@@ -417,96 +442,448 @@ struct PoleArgs {
     PoleParams pp;
 };
 
-template <int NWC, bool ACHS>
-__global__ __launch_bounds__(256) void gpp_oz_pole_kernel(PoleArgs a)
+// LDS of the STREAM form (occupied coefficients, per-wave sums) and of a
+// kernel with the BLOCKS form as well (plus the per-block sums)
+struct StreamLds {
+    double2 occ[kOccLds];
+    double red[4][16];
+};
+struct PoleLds {
+    double2 occ[kOccLds];
+    double red[4][16];
+    double val[16][256];     // [value][thread] of a workgroup sum
+    double part[16][16];
+};
+
+// Workgroup sum of V <= 16 values per thread, once per workgroup: each
+// wave's sum (fixed butterfly), then the four waves in order.  Thread q < V
+// returns the total of value q.
+template <int V>
+__device__ __forceinline__ double wave_block_sum(const double (&v)[V], double (&red)[4][16])
 {
-    constexpr int NWAVES = 4;
-    __shared__ double red[NWAVES][4 * NWC];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const bool want_ssx = a.nv > 0;
-    const int m = blockIdx.x / a.chunks, ch = blockIdx.x % a.chunks;
-    const int c_lo = (int)((int64_t)ch * a.C / a.chunks), c_hi = (int)((int64_t)(ch + 1) * a.C / a.chunks);
-    const bool ok = a.igp[m] >= 0;
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const double x = wave_sum(v[q]);
+        if (lane == 0) red[wave][q] = x;
+    }
+    __syncthreads();
+    double x = 0.0;
+    if (tid < V) x = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    return x;
+}
+
+// Fixed-order workgroup sum of V values per thread: 16 partial sums per
+// value over threads s, s + 16, ..., s + 240, then their sum.  Thread q < V
+// returns the total of value q.  Two barriers; L.val and L.part may be
+// reused after the call.
+template <int V>
+__device__ __forceinline__ double block_sum(const double (&v)[V], PoleLds &L)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < V; ++q) L.val[q][tid] = v[q];
+    __syncthreads();
+    if (tid < 16 * V) {
+        const int q = tid >> 4, s = tid & 15;
+        double x = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x += L.val[q][s + 16 * j];
+        L.part[q][s] = x;
+    }
+    __syncthreads();
+    double x = 0.0;
+    if (tid < V) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) x += L.part[tid][s];
+    }
+    return x;
+}
+
+// The W frequencies wq[0..W-1] of one element: the per-frequency steps of
+// pole_element's hoisted form, every frequency live.
+template <int W>
+__device__ __forceinline__ void pole_freqs(const PoleParams &pp, const double (&wq)[W], const double (&iq)[W],
+                                           double2 wt, double2 Bie, double2 BoOm, double2 Boie, bool want_ssx,
+                                           double2 (&sach)[W], double2 (&sasx)[W])
+{
+    const double tol1 = 1e-13 * pp.limitone, tol2 = 1e-13 * pp.to1;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const double wdr = wq[q] - wt.x;
+        const double wdiffr = pole_wdiffr(wdr, -wt.y);
+        double2 delw;
+        double delwr;
+        pole_delw(wt, wdr, rcp_fast<1>(wdiffr), delw, delwr);
+        // points within 1e-13 of a branch threshold take the IEEE quotient,
+        // so every branch decision is the reference's
+        const bool near = fabs(delwr - pp.limitone) <= tol1 || fabs(delwr - pp.to1) <= tol2;
+        if (__ballot(near)) {
+            if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
+        }
+        const bool m = (wdiffr > pp.limittwo) && (delwr < pp.limitone);
+        const double2 t = (m || !pp.mask_on) ? cmul(Bie, delw) : make_double2(0.0, 0.0);
+        sach[q] = cadd(sach[q], t);
+        if (want_ssx) {
+            double2 S = m ? make_double2(BoOm.x * iq[q], BoOm.y * iq[q]) : make_double2(0.0, 0.0);
+            const bool need2 = !m && delwr > pp.to1;
+            if (__ballot(need2)) {
+                if (need2) {
+                    const double2 z = make_double2(delw.x + 0.5, delw.y);
+                    const double sc = -0.25 * rcp_fast<1>(z.x * z.x + z.y * z.y);
+                    const double2 v = cmul(Boie, cmul(delw, cconj(z)));
+                    S = make_double2(sc * v.x, sc * v.y);
+                }
+            }
+            sasx[q] = cadd(sasx[q], S);
+        }
+    }
+}
+
+struct PoleElems {
+    double2 wt[kPoleK], Bie[kPoleK], BoOm[kPoleK], Boie[kPoleK];
+};
+
+// The W frequencies w0 .. w0 + W - 1 of the current column block's
+// elements, added to this thread's sums
+template <int W>
+__device__ __forceinline__ void oz_pole_freqs(const PoleArgs &a, int w0, bool want_ssx, const PoleElems &E,
+                                              double2 (&sach)[W], double2 (&sasx)[W])
+{
+    double wq[W], iq[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        wq[q] = a.pp.pc.wx[w0 + q];
+        iq[q] = a.pp.pc.iwx2[w0 + q];
+    }
+#pragma unroll
+    for (int e = 0; e < kPoleK; ++e)
+        pole_freqs<W>(a.pp, wq, iq, E.wt[e], E.Bie[e], E.BoOm[e], E.Boie[e], want_ssx, sach, sasx);
+}
+
+// workgroup total of the thread sums; thread q < 4 W returns value q
+// (frequency q / 4; component q % 4: achtemp re, im, asxtemp re, im)
+template <int W>
+__device__ __forceinline__ double oz_pole_total(const double2 (&sach)[W], const double2 (&sasx)[W], PoleLds &L)
+{
+    double v[4 * W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        v[4 * q + 0] = sach[q].x;
+        v[4 * q + 1] = sach[q].y;
+        v[4 * q + 2] = sasx[q].x;
+        v[4 * q + 3] = sasx[q].y;
+    }
+    return block_sum<4 * W>(v, L);
+}
+
+// One column block of row m: loads first (all independent), then Bo, then
+// the products every pass uses.  Columns past c_hi and skipped rows are
+// dead elements: B = Bo = 0 with wtilde = i, so every quotient is finite
+// (|wdiff|^2 >= 1) and every term they add is zero.
+template <bool ACHS>
+__device__ __forceinline__ void oz_pole_elems(const PoleArgs &a, const PoleLds &L, int m, bool ok, bool want_ssx,
+                                              int c0, int c_hi, PoleElems &E, double2 &sachs)
+{
+    const int tid = threadIdx.x;
     const double ram = a.ra[m];
     const double2 *brow = a.bmat + (size_t)m * a.ldn;
     const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
-    // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:
-    // 192-230), taken with the first frequency pass in the ACHS form
-    double2 sachs = make_double2(0.0, 0.0);
-    for (int w0 = 0; w0 < a.pp.nw; w0 += NWC) {
-        double2 sach[NWC], sasx[NWC];
+    bool live[kPoleK];
+    double2 Bu[kPoleK], wt[kPoleK], ie[kPoleK], Bo[kPoleK];
+    double rbv[kPoleK];
 #pragma unroll
-        for (int q = 0; q < NWC; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
-        if (ok) {
-            for (int ig = c_lo + tid; ig < c_hi; ig += 256) {
-                const double2 Bu = brow[ig];
-                const double sc = ram * a.rb[ig];
-                const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
-                double2 Bo = make_double2(0.0, 0.0);
-                for (int n1 = 0; n1 < a.nv; ++n1)
-                    Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
-                pole_element<NWC, true>(a.pp, w0, true, widen(wrow[ig]), widen(erow[ig]), B, Bo, want_ssx, sach,
-                                        sasx);
-                if constexpr (ACHS) {
-                    if (w0 == 0) sachs = csub(sachs, cmul(B, widen(erow[ig])));
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < NWC; ++q) {
-            double v0 = wave_sum(sach[q].x), v1 = wave_sum(sach[q].y);
-            double v2 = wave_sum(sasx[q].x), v3 = wave_sum(sasx[q].y);
-            if (lane == 0) {
-                red[wave][4 * q + 0] = v0;
-                red[wave][4 * q + 1] = v1;
-                red[wave][4 * q + 2] = v2;
-                red[wave][4 * q + 3] = v3;
-            }
-        }
-        __syncthreads();
-        if (tid < 4 * NWC) {
-            const int q = tid >> 2, comp = tid & 3;
-            const int iw = w0 + q;
-            if (iw < a.pp.nw) {
-                double sum = red[0][tid];
-#pragma unroll
-                for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
-                double *dst = reinterpret_cast<double *>(a.partials + ((size_t)blockIdx.x * a.pp.nw + iw) * 2);
-                dst[comp] = sum;
-            }
-        }
-        __syncthreads();
+    for (int e = 0; e < kPoleK; ++e) {
+        const int ig = c0 + e * 256 + tid;
+        live[e] = ok && ig < c_hi;
+        Bu[e] = live[e] ? brow[ig] : make_double2(0.0, 0.0);
+        rbv[e] = live[e] ? a.rb[ig] : 0.0;
+        wt[e] = live[e] ? widen(wrow[ig]) : make_double2(0.0, 1.0);
+        ie[e] = live[e] ? widen(erow[ig]) : make_double2(0.0, 0.0);
+        Bo[e] = make_double2(0.0, 0.0);
     }
-    if constexpr (ACHS) {
-        const double v0 = wave_sum(sachs.x), v1 = wave_sum(sachs.y);
-        if (lane == 0) {
-            red[wave][0] = v0;
-            red[wave][1] = v1;
-        }
-        __syncthreads();
-        if (tid < 2) {
-            double sum = red[0][tid];
+    if (ok && want_ssx) {
+        for (int n1 = 0; n1 < a.nv; ++n1) {
+            const double2 co = n1 < kOccLds ? L.occ[n1] : a.c_occ[(size_t)n1 * a.ldm + m];
+            const float2 *arow = a.aqsn + (size_t)n1 * a.C;
 #pragma unroll
-            for (int w = 1; w < NWAVES; ++w) sum += red[w][tid];
-            reinterpret_cast<double *>(a.achs_part + blockIdx.x)[tid] = sum;
+            for (int e = 0; e < kPoleK; ++e) {
+                const int ig = c0 + e * 256 + tid;
+                const double2 x = live[e] ? widen(arow[ig]) : make_double2(0.0, 0.0);
+                Bo[e] = cadd(Bo[e], cmul(co, x));
+            }
         }
+    }
+#pragma unroll
+    for (int e = 0; e < kPoleK; ++e) {
+        const double sc = ram * rbv[e];
+        const double2 B = make_double2(Bu[e].x * sc, Bu[e].y * sc);
+        const double2 Om = cmul(cmul(wt[e], wt[e]), ie[e]);
+        E.wt[e] = wt[e];
+        E.Bie[e] = cmul(B, ie[e]);
+        E.BoOm[e] = cmul(Bo[e], Om);
+        E.Boie[e] = cmul(Bo[e], ie[e]);
+        // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:192-230)
+        if constexpr (ACHS) sachs = csub(sachs, E.Bie[e]);
     }
 }
 
-constexpr int kPoleChunks = 4;   // column chunks per row of the pole kernel
-
-template <bool ACHS>
-void launch_pole(int nfreq, int blocks, hipStream_t s, const PoleArgs &pa)
+// nw <= 4 (one pass): the chunk's columns one element per thread and
+// iteration, each element's products formed and used at once (few live
+// registers, so many waves hide the loads); the thread sums are summed
+// over the workgroup once
+template <int W, bool ACHS, class LDS>
+__device__ __forceinline__ void oz_pole_stream(const PoleArgs &a, LDS &L, int b, int m, bool ok,
+                                               bool want_ssx, int c_lo, int c_hi, double2 &sachs)
 {
-    switch (std::min(nfreq, 4)) {
-    case 1: hipLaunchKernelGGL((gpp_oz_pole_kernel<1, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
-    case 2: hipLaunchKernelGGL((gpp_oz_pole_kernel<2, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
-    case 3: hipLaunchKernelGGL((gpp_oz_pole_kernel<3, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
-    default: hipLaunchKernelGGL((gpp_oz_pole_kernel<4, ACHS>), dim3(blocks), dim3(256), 0, s, pa); break;
+    double wq[W], iq[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        wq[q] = a.pp.pc.wx[q];
+        iq[q] = a.pp.pc.iwx2[q];
     }
+    double2 sach[W], sasx[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
+    const int tid = threadIdx.x;
+    if (ok) {
+        const double ram = a.ra[m];
+        const double2 *brow = a.bmat + (size_t)m * a.ldn;
+        const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
+        const int nvl = min(a.nv, kOccLds);
+        for (int ig = c_lo + tid; ig < c_hi; ig += 256) {
+            const double2 Bu = brow[ig];
+            const double sc = ram * a.rb[ig];
+            const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
+            const double2 wt = widen(wrow[ig]), ie = widen(erow[ig]);
+            double2 Bo = make_double2(0.0, 0.0);
+            for (int n1 = 0; n1 < nvl; ++n1) Bo = cadd(Bo, cmul(L.occ[n1], widen(a.aqsn[(size_t)n1 * a.C + ig])));
+            for (int n1 = nvl; n1 < a.nv; ++n1)
+                Bo = cadd(Bo, cmul(a.c_occ[(size_t)n1 * a.ldm + m], widen(a.aqsn[(size_t)n1 * a.C + ig])));
+            const double2 Om = cmul(cmul(wt, wt), ie);
+            const double2 Bie = cmul(B, ie);
+            pole_freqs<W>(a.pp, wq, iq, wt, Bie, cmul(Bo, Om), cmul(Bo, ie), want_ssx, sach, sasx);
+            // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:192-230)
+            if constexpr (ACHS) sachs = csub(sachs, Bie);
+        }
+    }
+    double v[4 * W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        v[4 * q + 0] = sach[q].x;
+        v[4 * q + 1] = sach[q].y;
+        v[4 * q + 2] = sasx[q].x;
+        v[4 * q + 3] = sasx[q].y;
+    }
+    const double x = wave_block_sum<4 * W>(v, L.red);
+    if (tid < 4 * W) reinterpret_cast<double *>(a.partials + ((size_t)b * W + (tid >> 2)) * 2)[tid & 3] = x;
+}
+
+// nw > 4: per block, each pass's sums are summed over the workgroup and
+// added to the thread that owns them (thread q < 4 W of pass p: own[p])
+template <int W>
+__device__ __forceinline__ void oz_pole_pass(const PoleArgs &a, int w0, bool want_ssx, const PoleElems &E,
+                                             PoleLds &L, double &own)
+{
+    double2 sach[W], sasx[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
+    oz_pole_freqs<W>(a, w0, want_ssx, E, sach, sasx);
+    own += oz_pole_total<W>(sach, sasx, L);
+}
+
+// workgroup b of the problem's launch: row b / chunks, the (b % chunks)-th
+// of the row's columns.  Forms: STREAM for nw <= 4, BLOCKS (blocks of
+// kPoleCols, near-equal passes) for nw > 4; a kernel built with both picks
+// by nw.
+constexpr int kPoleStream = 1, kPoleBlocks = 2;
+template <bool ACHS, int FORMS, class LDS>
+__device__ __forceinline__ void oz_pole_body(const PoleArgs &a, int b, LDS &L)
+{
+    const int tid = threadIdx.x;
+    const int m = __builtin_amdgcn_readfirstlane(b / a.chunks);
+    const int ch = __builtin_amdgcn_readfirstlane(b - m * a.chunks);
+    const int c_lo = (int)((int64_t)ch * a.C / a.chunks), c_hi = (int)((int64_t)(ch + 1) * a.C / a.chunks);
+    const bool ok = a.igp[m] >= 0;
+    const bool want_ssx = a.nv > 0;
+    const int nvl = min(a.nv, kOccLds);
+    for (int n1 = tid; n1 < nvl; n1 += 256) L.occ[n1] = a.c_occ[(size_t)n1 * a.ldm + m];
+    __syncthreads();
+    const int nw = a.pp.nw, passes = (nw + 3) / 4;
+    double2 sachs = make_double2(0.0, 0.0);
+
+    if ((FORMS & kPoleStream) && ((FORMS & kPoleBlocks) == 0 || nw <= 4)) {
+        switch (nw) {
+        case 1: oz_pole_stream<1, ACHS>(a, L, b, m, ok, want_ssx, c_lo, c_hi, sachs); break;
+        case 2: oz_pole_stream<2, ACHS>(a, L, b, m, ok, want_ssx, c_lo, c_hi, sachs); break;
+        case 3: oz_pole_stream<3, ACHS>(a, L, b, m, ok, want_ssx, c_lo, c_hi, sachs); break;
+        default: oz_pole_stream<4, ACHS>(a, L, b, m, ok, want_ssx, c_lo, c_hi, sachs); break;
+        }
+    } else if constexpr ((FORMS & kPoleBlocks) != 0) {
+        double own[4] = {0.0, 0.0, 0.0, 0.0};     // this thread's totals, one per pass
+        for (int c0 = c_lo; c0 < c_hi; c0 += kPoleCols) {
+            PoleElems E;
+            oz_pole_elems<ACHS>(a, L, m, ok, want_ssx, c0, c_hi, E, sachs);
+            for (int p = 0, w0 = 0; p < passes; ++p) {
+                const int W = (nw - w0 + (passes - p) - 1) / (passes - p);
+                switch (W) {
+                case 1: oz_pole_pass<1>(a, w0, want_ssx, E, L, own[p]); break;
+                case 2: oz_pole_pass<2>(a, w0, want_ssx, E, L, own[p]); break;
+                case 3: oz_pole_pass<3>(a, w0, want_ssx, E, L, own[p]); break;
+                default: oz_pole_pass<4>(a, w0, want_ssx, E, L, own[p]); break;
+                }
+                w0 += W;
+            }
+        }
+        for (int p = 0, w0 = 0; p < passes; ++p) {
+            const int W = (nw - w0 + (passes - p) - 1) / (passes - p);
+            if (tid < 4 * W) {
+                const int q = tid >> 2, comp = tid & 3;
+                reinterpret_cast<double *>(a.partials + ((size_t)b * nw + w0 + q) * 2)[comp] = own[p];
+            }
+            w0 += W;
+        }
+    }
+    if constexpr (ACHS) {
+        __syncthreads();     // L.red is free again
+        const double v[2] = {sachs.x, sachs.y};
+        const double x = wave_block_sum<2>(v, L.red);
+        if (tid < 2) reinterpret_cast<double *>(a.achs_part + b)[tid] = x;
+    }
+}
+
+template <bool ACHS, int FORMS>
+__device__ __forceinline__ void oz_pole_block(const PoleArgs &a, int b)
+{
+    if constexpr (FORMS == kPoleStream) {
+        __shared__ StreamLds L;
+        oz_pole_body<ACHS, FORMS>(a, b, L);
+    } else {
+        __shared__ PoleLds L;
+        oz_pole_body<ACHS, FORMS>(a, b, L);
+    }
+}
+
+#ifdef FK_POLE_WAVES
+#define FK_POLE_ATTR __attribute__((amdgpu_waves_per_eu(FK_POLE_WAVES, FK_POLE_WAVES)))
+#else
+#define FK_POLE_ATTR
+#endif
+
+template <bool ACHS, int FORMS>
+__global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_kernel(PoleArgs a)
+{
+    oz_pole_block<ACHS, FORMS>(a, blockIdx.x);
+}
+
+// column chunks of a row in the pole kernel (one workgroup each)
+#ifndef FK_POLE_CHUNKS
+#define FK_POLE_CHUNKS 4
+#endif
+inline int pole_chunks(int C) { return std::min(FK_POLE_CHUNKS, div_up(C, kPoleCols)); }
+
+// ---------------------------------------------------------------------------
+// Batches: one launch of each kernel above for many problems.  Every
+// problem has a record of its three kernels' arguments; block b of a batched
+// launch belongs to the problem p with first[p] <= b < first[p + 1] and runs
+// that problem's block b - first[p].  A problem's GEMM range starts at a
+// multiple of the XCD count, so its tile order keeps its XCD locality, and
+// the blocks padding a range return before any barrier.
+// ---------------------------------------------------------------------------
+struct OzRecord {
+    PrepArgs pr;
+    GemmArgs ga;
+    PoleArgs pa;
+    int gemm_tiles;          // tiles of the problem (its GEMM range may be longer)
+};
+
+// the problem whose range of a launch holds block b (first: n + 1 entries)
+__device__ __forceinline__ int batch_problem(const int *first, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ __launch_bounds__(256) void oz_prep_batch_kernel(const OzRecord *rec, const int *first, int n)
+{
+    const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
+    oz_prep_block(rec[p].pr, b - first[p]);
+}
+
+__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecord *rec, const int *first, int n)
+{
+    const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
+    const int local = b - first[p];
+    if (local >= rec[p].gemm_tiles) return;
+    oz_gemm_tile(rec[p].ga, local);
+}
+
+__global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
+{
+    const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
+    oz_pole_block<false, kPoleStream | kPoleBlocks>(rec[p].pa, b - first[p]);
 }
 
 }  // namespace
+
+size_t gpp_oz_batch_record_bytes() { return sizeof(OzRecord); }
+
+OzBlocks gpp_oz_batch_record(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, const Scratch &ws,
+                             const int *d_igp, void *record)
+{
+    OzRecord r;
+    PrepArgs &pr = r.pr;
+    pr.aqsm = static_cast<const float2 *>(p->aqsmtemp); pr.aqsn = static_cast<const float2 *>(p->aqsntemp);
+    pr.vcoul = p->vcoul; pr.igp = d_igp;
+    pr.as = static_cast<int8_t *>(ws.at(L.off_as));
+    pr.bs = static_cast<int8_t *>(ws.at(L.off_bs));
+    pr.ra = static_cast<double *>(ws.at(L.off_ea));
+    pr.rb = static_cast<double *>(ws.at(L.off_eb));
+    pr.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
+    pr.nb = p->number_bands; pr.nv = p->nvband; pr.C = p->ncouls; pr.G = p->ngpown;
+    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkh = L.Kp / kKHalf; pr.blocks_a = L.ldm / kPrepRowsA;
+    GemmArgs &ga = r.ga;
+    ga.as = pr.as; ga.bs = pr.bs;
+    ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
+    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = pr.nkh;
+    ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
+    PoleArgs &pa = r.pa;
+    pa.bmat = ga.bmat; pa.ra = pr.ra; pa.rb = pr.rb; pa.c_occ = pr.c_occ; pa.aqsn = pr.aqsn;
+    pa.I_eps = static_cast<const float2 *>(p->I_eps_array); pa.wtilde = static_cast<const float2 *>(p->wtilde_array);
+    pa.igp = d_igp;
+    pa.partials = static_cast<double2 *>(ws.at(L.off_part));
+    pa.achs_part = nullptr;
+    pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
+    pa.chunks = pole_chunks(p->ncouls);
+    pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
+    r.gemm_tiles = L.tiles_m * L.tiles_n;
+    std::memcpy(record, &r, sizeof(r));
+    OzBlocks nb;
+    const bool any = p->number_bands > 0;
+    nb.prep = any ? L.ldm / kPrepRowsA + L.ldn / kPrepColsB : 0;
+    nb.gemm = any ? div_up(r.gemm_tiles, kXcds) * kXcds : 0;
+    nb.pole = any ? p->ngpown * pa.chunks : 0;
+    return nb;
+}
+
+int gpp_oz_batch_launch(const void *records, const int *first_prep, const int *first_gemm, const int *first_pole,
+                        int n, int total_prep, int total_gemm, int total_pole, hipStream_t s)
+{
+    const OzRecord *rec = static_cast<const OzRecord *>(records);
+    if (total_prep > 0)
+        hipLaunchKernelGGL(oz_prep_batch_kernel, dim3(total_prep), dim3(256), 0, s, rec, first_prep, n);
+    if (total_gemm > 0)
+        hipLaunchKernelGGL(gpp_oz_gemm_batch_kernel, dim3(total_gemm), dim3(256), 0, s, rec, first_gemm, n);
+    if (total_pole > 0)
+        hipLaunchKernelGGL(gpp_oz_pole_batch_kernel, dim3(total_pole), dim3(256), 0, s, rec, first_pole, n);
+    FK_HIP(hipGetLastError());
+    return FK_OK;
+}
 
 void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
 {
@@ -522,8 +899,8 @@ void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
     L.off_eb = ws.plan(sizeof(double) * (size_t)L.ldn);
     L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
     L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
-    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * kPoleChunks);
-    L.off_achs = ws.plan(sizeof(double2) * (size_t)p->ngpown * kPoleChunks);
+    L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * pole_chunks(p->ncouls));
+    L.off_achs = ws.plan(sizeof(double2) * (size_t)p->ngpown * pole_chunks(p->ncouls));
 }
 
 int gpp_oz_check(const fk_gpp_problem *p)
@@ -570,11 +947,16 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
     pa.achs_part = d_achs ? static_cast<double2 *>(ws.at(L.off_achs)) : nullptr;
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;
-    pa.chunks = kPoleChunks;
+    pa.chunks = pole_chunks(p->ncouls);
     pa.pp = make_pole_params(p->wx, p->nfreq, p->limittwo, p->limitone, p->to1, o->mask != 0);
-    const int blocks = p->ngpown * kPoleChunks;
-    if (pa.achs_part) launch_pole<true>(p->nfreq, blocks, s, pa);
-    else launch_pole<false>(p->nfreq, blocks, s, pa);
+    const int blocks = p->ngpown * pa.chunks;
+    if (p->nfreq <= 4) {
+        if (pa.achs_part) hipLaunchKernelGGL((gpp_oz_pole_kernel<true, kPoleStream>), dim3(blocks), dim3(256), 0, s, pa);
+        else hipLaunchKernelGGL((gpp_oz_pole_kernel<false, kPoleStream>), dim3(blocks), dim3(256), 0, s, pa);
+    } else {
+        if (pa.achs_part) hipLaunchKernelGGL((gpp_oz_pole_kernel<true, kPoleBlocks>), dim3(blocks), dim3(256), 0, s, pa);
+        else hipLaunchKernelGGL((gpp_oz_pole_kernel<false, kPoleBlocks>), dim3(blocks), dim3(256), 0, s, pa);
+    }
     FK_HIP(hipGetLastError());
     FK_TRY(gpp_reduce_launch(pa.partials, blocks, p->nfreq, 2.0 * p->occ, d_ach, d_asx, s));
     if (d_achs) FK_TRY(gpp_sum_launch(pa.achs_part, blocks, d_achs, s));

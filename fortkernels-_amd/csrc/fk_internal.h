@@ -119,6 +119,20 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
                   const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
                   double2 *d_ach, double2 *d_asx, double2 *d_achs, hipStream_t s);
 
+// Batches of the int8-split path: each problem's kernel arguments as an
+// opaque record of gpp_oz_batch_record_bytes() bytes (written to `record`,
+// host memory) and its block counts; gpp_oz_batch_launch runs the prep,
+// GEMM and pole kernels of n records (device memory) as one launch each,
+// first_* holding each problem's first block (n + 1 ints, device memory).
+// The pole partials of a problem are [blocks.pole][nfreq][2] at the
+// layout's off_part, for the batched reduction.
+struct OzBlocks { int prep, gemm, pole; };
+size_t gpp_oz_batch_record_bytes();
+OzBlocks gpp_oz_batch_record(const fk_gpp_problem *p, const fk_options *o, const OzLayout &L, const Scratch &ws,
+                             const int *d_igp, void *record);
+int gpp_oz_batch_launch(const void *records, const int *first_prep, const int *first_gemm, const int *first_pole,
+                        int n, int total_prep, int total_gemm, int total_pole, hipStream_t s);
+
 // FF contraction in split BF16 on MFMA (FK_PREC_MIXED): workspace layout and
 // the launch of the slice pre-pass and the GEMM.  The GEMM writes FP64
 // partials [nf][nkc*ngt][nb][2] (schDt part, schDv part) for the reduction

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define FK_ABI_VERSION 2
+#define FK_ABI_VERSION 3
 
 typedef struct { double re, im; } fk_cdouble;  /* complex128 */
 typedef struct { float re, im; } fk_cfloat;    /* complex64 (mixed-precision storage) */
@@ -225,17 +225,20 @@ size_t gpp_kernel_noalloc_max_workspace(void);
  * on every run, so new values in the same arrays need no new batch.
  * Problems: nfreq <= 16; the arrays must outlive the batch.
  *
- * options->precision = FK_PREC_MIXED takes complex64 problems and runs the
- * persistent tile kernel with FP32 MFMA (configs as for FP64; error ~1e-6
- * against FP64 on the same values).  A form on the int8-split contraction
- * of gpp_kernel's mixed path (a graph of per-problem launches on parallel
- * branches) exists in the source but is switched off.
+ * options->precision = FK_PREC_MIXED takes complex64 problems.  With
+ * options->config = 0 it runs gpp_kernel's mixed path, the int8-split
+ * contraction, for every problem: its operand-slice, int8 GEMM and FP64
+ * pole kernels each as one launch over all problems' blocks, then one
+ * reduction (error below 1e-8 against FP64 on the same values; problems of
+ * at most 16000 bands).  With config 1..4 it runs the persistent tile kernel
+ * with FP32 MFMA in that configuration (error ~1e-6).
  *
  * gpp_batch_run launches the graph on `stream` and writes achtemp[i],
  * asxtemp[i] (device pointers, [nfreq_i] each); elapsed_ms (optional)
  * receives the graph's duration and synchronises.  Calls that share one
  * batch must be serialised by the caller.
- * gpp_kernel_batched = create (config 2 unless set) + run + synchronise + destroy.
+ * gpp_kernel_batched = create + run + synchronise + destroy, with config 2
+ * when options->config = 0 and the precision is FP64.
  * ------------------------------------------------------------------------ */
 typedef struct fk_gpp_batch fk_gpp_batch;
 
@@ -244,11 +247,8 @@ int gpp_batch_create(int32_t n, const fk_gpp_problem *const *problems, const fk_
 int gpp_batch_run(fk_gpp_batch *batch, fk_cdouble *const *achtemp, fk_cdouble *const *asxtemp, void *stream,
                   float *elapsed_ms);
 /* Tile configuration the batch uses (1-based, as fk_options.config; 0 for
- * the int8-split graph). */
+ * the int8-split contraction). */
 int gpp_batch_config(const fk_gpp_batch *batch);
-/* Branches of the int8-split graph (0 for the persistent tile kernel, which
- * every batch runs while that graph is switched off). */
-int gpp_batch_branches(const fk_gpp_batch *batch);
 size_t gpp_batch_workspace_bytes(const fk_gpp_batch *batch);
 void gpp_batch_destroy(fk_gpp_batch *batch);
 int gpp_kernel_batched(int32_t n, const fk_gpp_problem *const *problems, const fk_options *options,

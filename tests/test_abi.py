@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol(fk):
 
 
 def test_abi_version_and_target(fk):
-    assert fk.lib.fk_abi_version() == 2
+    assert fk.lib.fk_abi_version() == 3
     assert fk.lib.fk_build_target() == b"gfx950"
 
 

@@ -125,18 +125,22 @@ def test_batch64_two_band_slices_vs_oracle(fk, fko, problems64):
     b.close()
 
 
-def test_batch64_mixed(fk, fko, problems64):
-    """Mixed batch of the same 64 problems stored as complex64: each problem
-    against the FP64 batch on the same complex64 values (widened), and
+@pytest.mark.parametrize("config", [0, 2])
+def test_batch64_mixed(fk, fko, problems64, config):
+    """Mixed batch of the same 64 problems stored as complex64 (config 0: the
+    int8-split contraction; 2: the FP32 MFMA tile kernel): each problem
+    against FP64 gpp_kernel on the same complex64 values (widened), and
     against gpp_kernel's mixed result on the same configuration."""
     torch = _torch()
     lo = [dict(p, **{k: p[k].to(torch.complex64) for k in ("aqsm", "aqsn", "I_eps", "wtilde")})
           for p in problems64]
-    b = fk.GppBatch(lo, precision="mixed", config=0)
+    b = fk.GppBatch(lo, precision="mixed", config=config)
+    assert b.config == config
     ach, asx, ms = b.run(time_it=True)
     ach = [a.cpu().numpy() for a in ach]
     asx = [s.cpu().numpy() for s in asx]
-    print(f"mixed batch of 64: config {b.config}, branches {b.branches}, {ms:.1f} ms", flush=True)
+    print(f"mixed batch of 64: config {b.config}, {ms:.1f} ms, workspace {b.workspace_bytes / 2**20:.0f} MiB",
+          flush=True)
     worst_fp64, worst_single = 0.0, 0.0
     for k in range(0, 64, 4):
         p = lo[k]
