@@ -37,7 +37,8 @@ EXPORTS = ["fk_default_options", "gpp_kernel", "gpp_kernel_workspace_size", "ff_
            "gpp_kernel_ex", "fk_gpp_instance_write", "fk_ff_instance_write", "fk_instance_open",
            "fk_instance_kind", "fk_instance_complex_bytes", "fk_instance_gpp", "fk_instance_ff", "fk_instance_close",
            "fk_instance_list_read", "fk_instance_list_free", "ff_batch_create", "ff_batch_run", "ff_batch_config",
-           "ff_batch_workspace_bytes", "ff_batch_destroy", "gpp_kernel_noalloc", "gpp_kernel_noalloc_max_workspace"]
+           "ff_batch_workspace_bytes", "ff_batch_destroy", "gpp_kernel_noalloc", "gpp_kernel_noalloc_max_workspace",
+           "gpp_kernel_ex_workspace_size"]
 
 
 class FkError(RuntimeError):
@@ -161,6 +162,9 @@ def _load():
     L.ff_batch_destroy.argtypes = [P]
     L.gpp_kernel_ex.restype = ctypes.c_int
     L.gpp_kernel_ex.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options), ctypes.POINTER(GppOutputs)]
+    L.gpp_kernel_ex_workspace_size.restype = SZ
+    L.gpp_kernel_ex_workspace_size.argtypes = [ctypes.POINTER(GppProblem), ctypes.POINTER(Options),
+                                               ctypes.POINTER(GppOutputs)]
     L.ff_kernel_full.restype = ctypes.c_int
     L.ff_kernel_full.argtypes = [ctypes.POINTER(FfFullProblem), ctypes.POINTER(Options),
                                  ctypes.POINTER(FfFullOutputs)]
@@ -373,7 +377,9 @@ def gpp_kernel(nb, nv, ncouls, ngpown, wx, aqsm, aqsn, I_eps, wtilde, vcoul, inv
     return out + (achs,) if achstemp else out
 
 
-def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0):
+def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0, acht_n1=False):
+    """Device scratch bytes of gpp_kernel on this shape; acht_n1=True: of a
+    call that also returns acht_n1_loc (gpp_kernel_ex_workspace_size)."""
     wx = np.zeros(nfreq)
     dummy = 1  # non-NULL placeholders: only sizes are read
     inv = np.zeros(ngpown, np.int32)
@@ -383,6 +389,9 @@ def gpp_workspace_size(nb, nv, ncouls, ngpown, nfreq, precision="fp64", config=0
     opt = default_options()
     opt.precision = _precision(precision)
     opt.config = config
+    if acht_n1:
+        outs = GppOutputs(dummy, dummy, dummy, None)
+        return int(lib.gpp_kernel_ex_workspace_size(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(outs)))
     return int(lib.gpp_kernel_workspace_size(ctypes.byref(prob), ctypes.byref(opt)))
 
 

@@ -594,19 +594,38 @@ int ff_launch(const fk_ff_problem *p, const fk_options *o, const FfLayout &L, Sc
     return timer.finish();
 }
 
-int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
-                     const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
-                     hipStream_t s)
+namespace {
+fk_ff_problem rows_problem(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+                           const double *vcoul)
 {
     fk_ff_problem p;
     p.number_bands = nb; p.ncouls = C; p.ngpown = G; p.nfreq = 1;
     p.aqsmtemp = aqsm; p.aqsntemp = aqsn; p.I_eps_array = X; p.vcoul = vcoul;
     p.inv_igp_index = nullptr; p.indinv = nullptr; p.achs_freq = 0;
+    return p;
+}
+}  // namespace
+
+size_t ff_rows_weighted_bytes(int nb, int C, int G)
+{
+    const fk_ff_problem p = rows_problem(nb, C, G, nullptr, nullptr, nullptr, nullptr);
+    fk_options o = fk_default_options();
+    o.memory = FK_MEM_DEVICE;
+    Scratch ws;
+    ff_plan(&p, &o, ws);
+    return ws.total();
+}
+
+int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
+                     const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
+                     void *wsp, size_t ws_bytes, hipStream_t s)
+{
+    const fk_ff_problem p = rows_problem(nb, C, G, aqsm, aqsn, X, vcoul);
     fk_options o = fk_default_options();
     o.memory = FK_MEM_DEVICE;
     Scratch ws;
     FfLayout L = ff_plan(&p, &o, ws);
-    FK_TRY(ws.acquire(nullptr, 0, s));
+    FK_TRY(ws.acquire(wsp, ws_bytes, s));
     return ff_launch(&p, &o, L, ws, cols, aqsm, aqsn, X, vcoul, nullptr, out, nullptr, vscale, s);
 }
 

@@ -590,9 +590,12 @@ __global__ void widen_kernel(const float2 *src, int64_t n, double2 *dst)
 // host side
 // ---------------------------------------------------------------------------
 #ifndef FK_GPP_GROUP_M
-#define FK_GPP_GROUP_M 1 << 20
+#define FK_GPP_GROUP_M 4
 #endif
-// row tiles per band of the fused kernel's tile order (tile_coords)
+// row tiles per band of the fused kernel's tile order (tile_coords).  At
+// 512/2/32768/20 (26 x 512 tiles of 64 x 64) FETCH_SIZE per launch, x2 to
+// bytes: whole columns 7.91 GB, bands of 4 or 8 row tiles 4.44 GB, of 13
+// 5.3 GB (algorithmic 2.00 GB); the kernel time is the same for all.
 constexpr int kGroupM = FK_GPP_GROUP_M;
 
 struct TileCfg { int bm, bn, p, minw; };
@@ -798,21 +801,48 @@ int launch_gpp(const fk_gpp_problem *p, const fk_options *o, const Layout &L, Sc
     return timer.finish();
 }
 
+// Scratch of the per-band output: X [G][C], widened aqsm and aqsn
+// (complex64 inputs) and the row contraction's own scratch
+struct N1Layout {
+    size_t off_x = 0, off_m = 0, off_n = 0, off_rows = 0, rows_bytes = 0;
+};
+N1Layout plan_n1(const fk_gpp_problem *p, bool wide, Scratch &ws)
+{
+    const int nb = std::max(p->number_bands, 1), C = p->ncouls, G = p->ngpown;
+    N1Layout N;
+    N.off_x = ws.plan(sizeof(double2) * (size_t)G * C);
+    if (!wide) {
+        N.off_m = ws.plan(sizeof(double2) * (size_t)nb * C);
+        N.off_n = ws.plan(sizeof(double2) * (size_t)nb * C);
+    }
+    N.rows_bytes = ff_rows_weighted_bytes(nb, C, G);
+    N.off_rows = ws.plan(N.rows_bytes);
+    return N;
+}
+size_t n1_bytes(const fk_gpp_problem *p, bool wide)
+{
+    Scratch ws;
+    plan_n1(p, wide, ws);
+    return ws.total();
+}
+
 // acht_n1_loc[n1] = sum_my_igp 0.5 vcoul[igp] conj(aqsm[n1][igp]) sum_ig aqsn[n1][ig] X[my_igp][ig]
 // with X the pole factor at frequency index 2 (BGW-Kernels/gppKer.cpp:424):
 // one more complex GEMM over all bands, run as the FF row contraction.
+// Scratch: `base` (n1_bytes of caller workspace) or, with base = NULL, a
+// stream-ordered allocation for the call.
 template <class E>
 int launch_acht_n1(const fk_gpp_problem *p, const fk_options *o, const std::vector<int32_t> &cols, const E *aqsm,
-                   const E *aqsn, const E *ieps, const E *wt, const double *vcoul, double2 *d_n1, hipStream_t s)
+                   const E *aqsn, const E *ieps, const E *wt, const double *vcoul, double2 *d_n1, void *base,
+                   hipStream_t s)
 {
     const int nb = p->number_bands, C = p->ncouls, G = p->ngpown;
     if (nb == 0) return FK_OK;
     const bool wide = sizeof(E) == sizeof(double2);
     Scratch ws;
-    const size_t off_x = ws.plan(sizeof(double2) * (size_t)G * C);
-    const size_t off_m = wide ? 0 : ws.plan(sizeof(double2) * (size_t)nb * C);
-    const size_t off_n = wide ? 0 : ws.plan(sizeof(double2) * (size_t)nb * C);
-    FK_TRY(ws.acquire(nullptr, 0, s));
+    const N1Layout N = plan_n1(p, wide, ws);
+    const size_t off_x = N.off_x, off_m = N.off_m, off_n = N.off_n;
+    FK_TRY(ws.acquire(base, base ? ws.total() : 0, s));
     double2 *X = static_cast<double2 *>(ws.at(off_x));
     const int64_t gc = (int64_t)G * C;
     const int blocks = (int)std::min<int64_t>((gc + 255) / 256, 8192);
@@ -833,13 +863,21 @@ int launch_acht_n1(const fk_gpp_problem *p, const fk_options *o, const std::vect
         am = wm;
         an = wn;
     }
-    return ff_rows_weighted(nb, C, G, am, an, X, vcoul, cols, 0.5, d_n1, s);
+    return ff_rows_weighted(nb, C, G, am, an, X, vcoul, cols, 0.5, d_n1, ws.at(N.off_rows), N.rows_bytes, s);
+}
+
+// The part of the caller's workspace after its first `used` bytes, when it
+// holds `need` bytes more; else NULL.
+void *workspace_tail(const fk_options *o, size_t used, size_t need)
+{
+    if (!o->workspace || need == 0 || o->workspace_bytes < used + need) return nullptr;
+    return static_cast<char *>(o->workspace) + used;
 }
 
 // Host-memory path: upload, run, download (synchronous).
 template <class E, class Launch>
 int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int32_t> &cols, double2 *ach,
-             double2 *asx, double2 *n1, double2 *achs, hipStream_t s, Launch &&launch)
+             double2 *asx, double2 *n1, double2 *achs, void *n1_base, hipStream_t s, Launch &&launch)
 {
     const size_t nbC = (size_t)p->number_bands * p->ncouls, GC = (size_t)p->ngpown * p->ncouls;
     E *dm = nullptr, *dn = nullptr, *die = nullptr, *dwt = nullptr;
@@ -870,7 +908,7 @@ int run_host(const fk_gpp_problem *p, const fk_options *o, const std::vector<int
     double2 *d_achs = achs ? dout + 2 * p->nfreq : nullptr;
     double2 *d_n1 = dout + 2 * p->nfreq + 1;
     if (rc == FK_OK) rc = launch(dm, dn, die, dwt, dv, dout, dout + p->nfreq, d_achs);
-    if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, d_n1, s);
+    if (rc == FK_OK && n1) rc = launch_acht_n1<E>(p, o, cols, dm, dn, die, dwt, dv, d_n1, n1_base, s);
     if (rc == FK_OK) {
         hipError_t e = hipMemcpyAsync(ach, dout, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipMemcpyAsync(asx, dout + p->nfreq, sizeof(double2) * p->nfreq, hipMemcpyDeviceToHost, s);
@@ -1431,6 +1469,15 @@ size_t gpp_kernel_workspace_size(const fk_gpp_problem *p, const fk_options *o)
     return ws.total();
 }
 
+size_t gpp_kernel_ex_workspace_size(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_outputs *out)
+{
+    fk_options d = fk_default_options();
+    if (!o) o = &d;
+    const size_t base = gpp_kernel_workspace_size(p, o);
+    if (base == 0 || !out || !out->acht_n1_loc || p->semantics != FK_GPP_CANONICAL) return base;
+    return base + n1_bytes(p, o->precision != FK_PREC_MIXED);
+}
+
 int gpp_kernel(const fk_gpp_problem *p, const fk_options *o, fk_cdouble *achtemp, fk_cdouble *asxtemp,
                fk_cdouble *acht_n1_loc)
 {
@@ -1480,6 +1527,7 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
         OzLayout Z;
         gpp_oz_plan(p, ws, Z);
         const size_t off_igp = ws.plan(sizeof(int) * (size_t)p->ngpown);
+        void *n1_base = workspace_tail(o, ws.total(), n1 ? n1_bytes(p, false) : 0);
         rc = ws.acquire(o->workspace, o->workspace_bytes, s);
         int *d_igp = static_cast<int *>(ws.at(off_igp));
         if (rc == FK_OK) {
@@ -1500,14 +1548,16 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
                 const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
                 const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
                 rc = launch(m, n, ie, w, p->vcoul, ach, asx, achs);
-                if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                if (rc == FK_OK && n1)
+                    rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, n1_base, s);
             } else {
-                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, s, launch);
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, n1_base, s, launch);
             }
         }
     } else {
         Scratch ws;
         Layout L = plan_layout(p, o, ws, mixed ? sizeof(float2) : sizeof(double2));
+        void *n1_base = workspace_tail(o, ws.total(), n1 ? n1_bytes(p, !mixed) : 0);
         rc = ws.acquire(o->workspace, o->workspace_bytes, s);
         if (rc == FK_OK) {
             if (o->memory == FK_MEM_DEVICE) {
@@ -1515,7 +1565,8 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
                     const float2 *m = (const float2 *)p->aqsmtemp, *n = (const float2 *)p->aqsntemp;
                     const float2 *ie = (const float2 *)p->I_eps_array, *w = (const float2 *)p->wtilde_array;
                     rc = launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
-                    if (rc == FK_OK && n1) rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                    if (rc == FK_OK && n1)
+                        rc = launch_acht_n1<float2>(p, o, igp, m, n, ie, w, p->vcoul, n1, n1_base, s);
                 } else {
                     const double2 *m = (const double2 *)p->aqsmtemp, *n = (const double2 *)p->aqsntemp;
                     const double2 *ie = (const double2 *)p->I_eps_array, *w = (const double2 *)p->wtilde_array;
@@ -1523,18 +1574,19 @@ int gpp_kernel_ex(const fk_gpp_problem *p, const fk_options *o, const fk_gpp_out
                         rc = launch_gpp_sym(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, s);
                     } else {
                         rc = launch_gpp<double2>(p, o, L, ws, igp, m, n, ie, w, p->vcoul, ach, asx, achs, s);
-                        if (rc == FK_OK && n1) rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, s);
+                        if (rc == FK_OK && n1)
+                            rc = launch_acht_n1<double2>(p, o, igp, m, n, ie, w, p->vcoul, n1, n1_base, s);
                     }
                 }
             } else if (mixed) {
-                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, s,
+                rc = run_host<float2>(p, o, igp, ach, asx, n1, achs, n1_base, s,
                                       [&](const float2 *m, const float2 *n, const float2 *ie, const float2 *w,
                                           const double *v, double2 *da, double2 *dx, double2 *dc) {
                                           return launch_gpp<float2>(p, o, L, ws, igp, m, n, ie, w, v, da, dx, dc,
                                                                     s);
                                       });
             } else {
-                rc = run_host<double2>(p, o, igp, ach, asx, n1, achs, s,
+                rc = run_host<double2>(p, o, igp, ach, asx, n1, achs, n1_base, s,
                                        [&](const double2 *m, const double2 *n, const double2 *ie, const double2 *w,
                                            const double *v, double2 *da, double2 *dx, double2 *dc) {
                                            if (sym) return launch_gpp_sym(p, o, L, ws, igp, m, n, ie, w, v, da, dx, s);

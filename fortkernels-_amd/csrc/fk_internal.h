@@ -85,10 +85,12 @@ private:
 // output of gpp_kernel:
 //   out[n1] = vscale * sum_m vcoul[igp_m] conj(aqsm[n1][igp_m]) sum_ig aqsn[n1][ig] X[m][ig]
 // for one X[G][C] (igp_m = cols[m], -1 = skipped column).  Asynchronous on
-// `s`; its scratch is a stream-ordered allocation.
+// `s`; its scratch is `ws` (at least ff_rows_weighted_bytes) or, with
+// ws = NULL, a stream-ordered allocation.
+size_t ff_rows_weighted_bytes(int nb, int C, int G);
 int ff_rows_weighted(int nb, int C, int G, const double2 *aqsm, const double2 *aqsn, const double2 *X,
                      const double *vcoul, const std::vector<int32_t> &cols, double vscale, double2 *out,
-                     hipStream_t s);
+                     void *ws, size_t ws_bytes, hipStream_t s);
 
 // Fixed-order sum of GPP per-tile partials [tiles][nw][2] into achtemp and
 // asxtemp (asxtemp scaled by asx_scale).

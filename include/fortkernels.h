@@ -168,7 +168,10 @@ int gpp_kernel(const fk_gpp_problem *problem, const fk_options *options,
                fk_cdouble *achtemp, fk_cdouble *asxtemp, fk_cdouble *acht_n1_loc);
 
 /* Device scratch bytes gpp_kernel (and gpp_kernel_ex) needs for this
- * problem and options. */
+ * problem and options, for achtemp, asxtemp and achstemp.  The per-band
+ * output acht_n1_loc needs more (its pole factors at iw = 2, G x ncouls
+ * complex doubles, and a second contraction): see
+ * gpp_kernel_ex_workspace_size. */
 size_t gpp_kernel_workspace_size(const fk_gpp_problem *problem, const fk_options *options);
 
 /* gpp_kernel with its outputs in one struct and one more output, the
@@ -190,6 +193,16 @@ typedef struct {
 } fk_gpp_outputs;
 
 int gpp_kernel_ex(const fk_gpp_problem *problem, const fk_options *options, const fk_gpp_outputs *outputs);
+
+/* Device scratch bytes for a gpp_kernel_ex call with these outputs (only
+ * which pointers are NULL matters; outputs may be NULL for achtemp and
+ * asxtemp alone).  A caller workspace of at least this size serves the whole
+ * call, acht_n1_loc included; a smaller one of at least
+ * gpp_kernel_workspace_size() serves the rest, and acht_n1_loc then takes a
+ * stream-ordered allocation of its own for the call.  No counterpart in
+ * the reference, which allocates nothing on a device. */
+size_t gpp_kernel_ex_workspace_size(const fk_gpp_problem *problem, const fk_options *options,
+                                    const fk_gpp_outputs *outputs);
 
 /* gpp_kernel that allocates nothing.  No counterpart in the reference; it
  * is for callers that cannot allocate device memory (code under graph

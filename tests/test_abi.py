@@ -73,6 +73,19 @@ def test_gpp_workspace_size_grows_with_problem(fk):
     assert big < 64 << 20   # coefficient matrix + per-tile partials only
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpp_workspace_size_with_acht_n1(fk, precision):
+    """gpp_kernel_ex_workspace_size adds the per-band output's scratch (its
+    G x ncouls pole factors, the widened band arrays of complex64 inputs,
+    the row contraction's own) only when acht_n1_loc is asked for."""
+    nb, C, G = 512, 32768, 1638
+    base = fk.gpp_workspace_size(nb, 2, C, G, 3, precision=precision)
+    n1 = fk.gpp_workspace_size(nb, 2, C, G, 3, precision=precision, acht_n1=True)
+    extra = 16 * G * C + (2 * 16 * nb * C if precision == "mixed" else 0)
+    assert n1 >= base + extra
+    assert n1 < base + extra + (64 << 20)
+
+
 def test_complex_helpers_host(fk):
     rng = np.random.default_rng(0)
     a = rng.normal(size=64) + 1j * rng.normal(size=64)

@@ -126,3 +126,38 @@ def test_n1_full_size_sums_to_achtemp(fk, fko):
                        t["wtilde"].view(G, C).cpu().numpy(), v.cpu().numpy(), inv, indinv)
     _, _, ref = fko.gpp(p, want_n1loc=True, nthreads=8)
     _close(n1.cpu().numpy()[sel], ref, what="full-size sampled bands")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_n1_in_caller_workspace(fk, fko, precision):
+    """A caller workspace of gpp_workspace_size(acht_n1=True) bytes serves the
+    per-band output as well (no allocation of its own); one of the smaller
+    gpp_workspace_size() leaves that output its per-call allocation.  Both
+    give the outputs of a call without a workspace, bit for bit, and a
+    workspace filled with NaN bytes changes nothing."""
+    torch = _torch()
+    nb, nv, C, G = 96, 3, 4096, 205
+    p = fko.gpp_random(nb, nv, C, G, seed=33, wtilde_re=(3.0, 5.0))
+    cdt = torch.complex64 if precision == "mixed" else torch.complex128
+    t = _dev(p, cdt)
+    big = fk.gpp_workspace_size(nb, nv, C, G, len(p.wx), precision=precision, acht_n1=True)
+    small = fk.gpp_workspace_size(nb, nv, C, G, len(p.wx), precision=precision)
+    assert big >= small + 16 * G * C
+    print(f"acht_n1 workspace {precision}: {small} B without, {big} B with the per-band output", flush=True)
+
+    def call(ws):
+        r = fk.gpp_kernel(nb, nv, C, G, p.wx, t["aqsm"], t["aqsn"], t["I_eps"], t["wtilde"], t["vcoul"],
+                          p.inv_igp_index, p.indinv, precision=precision, acht_n1=True, workspace=ws)
+        torch.cuda.synchronize()
+        return [x.cpu().numpy() for x in (r[0], r[1], r[3])]
+
+    ref = call(None)
+    for size in (big, small):
+        ws = torch.full((size,), 255, dtype=torch.uint8, device="cuda:0")
+        got = call(ws)
+        for g, r, what in zip(got, ref, ("achtemp", "asxtemp", "acht_n1_loc")):
+            assert np.array_equal(g, r), f"{precision} workspace {size} B: {what}"
+    _, _, n1ref = fko.gpp(p, want_n1loc=True)
+    if precision == "fp64":
+        _close(ref[2], n1ref, what="acht_n1_loc with a caller workspace")
+
