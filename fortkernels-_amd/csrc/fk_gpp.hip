@@ -475,7 +475,10 @@ __global__ __launch_bounds__(64 * WGM * WGN, MINW) void gpp_fused_kernel(GppArgs
 
 // Many problems in one launch: a fixed grid of workgroups takes tiles from a
 // shared counter, in the order of the tile table (largest K first), until
-// the table is exhausted.  tiles[t] = (problem, tm, tn).
+// the table is exhausted.  tiles[t] = (problem, tm, tn).  (Measured on the
+// 64-problem FP64 batch, 115.5 ms in this form: per-XCD segments of the
+// table with a counter each, 119.5 ms; tiles in bands of 4 row tiles, 116.8
+// ms; the second workgroup of each CU started half a tile late, 115.9 ms.)
 struct BatchTile { int prob, tm, tn; };
 
 template <class E, int MB, int NB, int P, int WGM, int WGN, int MINW, int NWC>
@@ -618,10 +621,22 @@ void launch_fused_nw(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
     }
 }
 
+// Frequencies per epilogue pass that the kernels hold in registers without
+// scratch (gfx950 build, checked with -Rpass-analysis=kernel-resource-usage):
+// FP64 configurations 0, 2 and 3 spill 800 B per lane at 4; the FP32 ones 2
+// and 3 spill 270-360 B at 1..3 and less or none at 4.
+template <class E> struct PassWidth;
+template <> struct PassWidth<double2> {
+    static constexpr int lo[kNumCfgs] = {1, 1, 1, 1}, hi[kNumCfgs] = {3, 4, 3, 3};
+};
+template <> struct PassWidth<float2> {
+    static constexpr int lo[kNumCfgs] = {1, 1, 4, 4}, hi[kNumCfgs] = {4, 4, 4, 4};
+};
+
 template <class E, bool ACHS>
 void launch_fused_achs(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 {
-    switch (a.nw) {
+    switch (std::max(PassWidth<E>::lo[cfg], std::min(a.nw, PassWidth<E>::hi[cfg]))) {
     case 1: launch_fused_nw<E, 1, ACHS>(cfg, tiles, s, a); break;
     case 2: launch_fused_nw<E, 2, ACHS>(cfg, tiles, s, a); break;
     case 3: launch_fused_nw<E, 3, ACHS>(cfg, tiles, s, a); break;
@@ -629,8 +644,9 @@ void launch_fused_achs(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
     }
 }
 
-// Frequencies per epilogue pass: all of them up to 4, else passes of 4; the
-// static-CH remainder only in the kernels of calls that ask for it.
+// Frequencies per epilogue pass: all of them up to the configuration's
+// width, else passes of that width; the static-CH remainder only in the
+// kernels of calls that ask for it.
 template <class E>
 void launch_fused(int cfg, int tiles, hipStream_t s, const GppArgs<E> &a)
 {
@@ -1020,8 +1036,9 @@ __global__ void batch_scatter_kernel(const double2 *out, const int *nw, double2 
 }
 
 // frequencies per epilogue pass in batches (3..11 frequencies: 4 takes the
-// mixed batch from 93 to 87 ms, the FP64 one unchanged at 136 ms)
-constexpr int kBatchNwc = 4;
+// mixed batch from 93 to 87 ms; the FP64 batch takes the same time with 2, 3
+// or 4), 3 for the FP64 128 x 64 tiles, which spill 800 B per lane at 4
+template <class E> constexpr int batch_nwc(int cfg) { return sizeof(E) == sizeof(double2) && cfg == 2 ? 3 : 4; }
 
 template <class E>
 void launch_batch(int cfg, int grid, hipStream_t s, const GppArgs<E> *probs, const BatchTile *tiles, int ntiles,
@@ -1029,20 +1046,20 @@ void launch_batch(int cfg, int grid, hipStream_t s, const GppArgs<E> *probs, con
 {
     switch (cfg) {
     case 0:
-        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 4, 1, 4, 1, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
-                           tiles, ntiles, counter);
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 4, 1, 4, 1, batch_nwc<E>(0)>), dim3(grid), dim3(256), 0, s,
+                           probs, tiles, ntiles, counter);
         break;
     case 1:
-        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 2, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
-                           tiles, ntiles, counter);
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 2, 3, 2, 2, 2, batch_nwc<E>(1)>), dim3(grid), dim3(256), 0, s,
+                           probs, tiles, ntiles, counter);
         break;
     case 2:
-        hipLaunchKernelGGL((gpp_batch_kernel<E, 4, 2, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
-                           tiles, ntiles, counter);
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 4, 2, 3, 2, 2, 2, batch_nwc<E>(2)>), dim3(grid), dim3(256), 0, s,
+                           probs, tiles, ntiles, counter);
         break;
     default:
-        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 3, 2, 2, 2, kBatchNwc>), dim3(grid), dim3(256), 0, s, probs,
-                           tiles, ntiles, counter);
+        hipLaunchKernelGGL((gpp_batch_kernel<E, 2, 4, 3, 2, 2, 2, batch_nwc<E>(3)>), dim3(grid), dim3(256), 0, s,
+                           probs, tiles, ntiles, counter);
         break;
     }
 }
