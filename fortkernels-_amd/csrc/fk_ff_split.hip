@@ -9,28 +9,32 @@
 // on v_mfma_f32_32x32x16_bf16 with FP32 accumulation; the three dropped
 // products are below 2^-24 of |x||y|.
 //
-// FP64 refinement: a workgroup accumulates one chunk of ig (1024 by default)
-// in FP32 and contracts its tile in FP64,
+// Accuracy is FP32-class: the operands are rounded to FP32 and T is summed
+// in FP32 over one chunk of ig (1024 by default), about 7e-7 normwise
+// against FP64 at 256/4096/4096/32.  What runs in FP64 is everything after
+// the chunk sums: a workgroup contracts its tile with the coefficients,
 //     schDt part[n1] = sum_m cf[n1][m] T[n1][m],   schDv part[n1] = sum_m vg[m] cf[n1][m] T[n1][m],
-// and the chunks and column tiles are summed in FP64, in a fixed order, by
-// the reduction of fk_ff.hip.  FP32 rounding therefore spans one chunk only.
+// and the chunks and column tiles are summed in a fixed order by the
+// reduction of fk_ff.hip, so FP32 rounding spans one chunk only.
 //
-// The complex product as two real GEMMs over K' = 2*ncouls with re and im of
-// each ig interleaved:
-//     A[n1][2ig] = Re a,    A[n1][2ig+1] = Im a                  (a = aqsn)
-//     B1[m][2ig] = Re x,    B1[m][2ig+1] = -Im x     Re T = A B1^T
-//     B2[m][2ig] = Im x,    B2[m][2ig+1] = Re x      Im T = A B2^T
-// X is read in its own layout; each slice dword (Re x_s, Im x_s) gives B1 by
-// flipping the sign bit of its high half and B2 by swapping its halves.
+// The complex product as three real GEMMs over K = ncouls (the 3M form):
+//     k1 = sum_ig (ar + ai) xr,   k2 = sum_ig ar (xi - xr),   k3 = sum_ig ai (xr + xi),
+//     Re T = k1 - k3,   Im T = k1 + k2                              (a = aqsn, x = X)
+// with the sums and differences formed in FP64 and rounded to FP32 before
+// the split.  Per 32 ig this is 3 x 2 MFMA k-steps of K = 16 where the
+// product as two real GEMMs over an interleaved K' = 2 ncouls takes
+// 2 x 4; the error stays at the FP32 level of the split.
 //
 // Tile: 256 bands x 64 columns of one frequency x one ig chunk per
-// workgroup of 4 waves; wave w owns 64 bands (2 x 2 blocks of 32x32, re and
-// im: 128 FP32 accumulators).  X is read from HBM once: 32 ig per stage,
-// split in registers and written as BF16 slices (B1 and B2) to a
-// double-buffered LDS stage that the four waves share.  Aqsn is split once
-// by ff_split_a_kernel into MFMA fragment order, and each wave loads its own
-// fragments from L2 one stage ahead.  The MFMAs of a k-step run in six
-// groups of eight; the splitting of X and the loads ride between them.
+// workgroup of 4 waves; wave w owns 64 bands (2 x 2 blocks of 32x32, one
+// accumulator per product: 192 FP32 accumulators).  X is read from HBM
+// once: 32 ig per stage, two consecutive ig of a column per lane, split in
+// registers into the three B matrices' slices (two ig per dword) and
+// written to a double-buffered LDS stage that the four waves share.  Aqsn
+// is split once by ff_split_a_kernel into the three A matrices in MFMA
+// fragment order, and each wave loads its own fragments from L2 one stage
+// ahead.  The MFMAs of a stage run in six groups of 24 (k-step, product);
+// the splitting of X and the loads ride between them.
 #include "fk_complex.h"
 #include "fk_internal.h"
 
@@ -44,14 +48,15 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kXcds = 8;
 constexpr int kSlices = 3;
-constexpr int kStageIg = 32;                 // ig per stage (K' = 64)
-constexpr int kSteps = 4;                    // MFMA k-steps (K' = 16) per stage
+constexpr int kProds = 3;                    // k1, k2, k3
+constexpr int kStageIg = 32;                 // ig per stage
+constexpr int kSteps = 2;                    // MFMA k-steps (K = 16 ig) per stage
 constexpr int kTileN = 64;                   // columns per workgroup
 constexpr int kTileM = 256;                  // bands per workgroup
 constexpr int kWaves = 4;
-constexpr int kRowDw = 36;                   // LDS row of one slice: 32 dwords + 4 pad
-constexpr int kSliceDw = kTileN * kRowDw;    // one slice of B1 or B2 for one stage
-constexpr int kBufDw = 2 * kSlices * kSliceDw;   // one stage: [B1, B2][slice][column][ig]
+constexpr int kRowDw = 20;                   // LDS row of one slice: 16 dwords (32 ig) + 4 pad
+constexpr int kSliceDw = kTileN * kRowDw;    // one slice of one B matrix for one stage
+constexpr int kBufDw = kProds * kSlices * kSliceDw;   // one stage: [product][slice][column][ig pair]
 constexpr int kEpiRow = 33;                  // epilogue row: 32 float2 + 1 pad
 constexpr int kEpiWave = 64 * kEpiRow;       // float2 per wave
 constexpr int kLdsBytes = (2 * kBufDw * 4 > kWaves * kEpiWave * 8) ? 2 * kBufDw * 4 : kWaves * kEpiWave * 8;
@@ -64,7 +69,8 @@ __device__ __forceinline__ uint32_t cvt_pk_bf16(float lo, float hi)
     return d;
 }
 
-// (x, y) -> three slice dwords (x_s, y_s), s = 0, 1, 2
+// (x, y) -> three slice dwords (x_s in the low half, y_s in the high half),
+// s = 0, 1, 2
 __device__ __forceinline__ void split3(float x, float y, uint32_t (&d)[kSlices])
 {
 #pragma unroll
@@ -83,8 +89,9 @@ __device__ __forceinline__ f32x16 mfma(uint4 a, uint4 b, f32x16 c)
                                                    0, 0, 0);
 }
 
-// As[((q*RF + r)*3 + s)*64 + L] (16 bytes): slice s of A for fragment row
-// 32r + L%32 and K' = 16q + 8(L/32) .. +7.  Zero past nb and past ncouls.
+// As[(((q*RF + r)*3 + p)*3 + s)*64 + L] (16 bytes): slice s of A matrix p
+// (ar + ai, ar, ai) for fragment row 32r + L%32 and ig = 16q + 8(L/32) ..
+// +7, two ig per dword.  Zero past nb and past ncouls.
 __global__ void ff_split_a_kernel(const double2 *aqsn, int nb, int C, int nq, int RF, uint4 *As)
 {
     const int64_t n = (int64_t)nq * RF * 64;
@@ -94,18 +101,27 @@ __global__ void ff_split_a_kernel(const double2 *aqsn, int nb, int C, int nq, in
         const int r = (int)((idx / 64) % RF);
         const int q = (int)(idx / (64 * RF));
         const int row = 32 * r + (L & 31);
-        const int ig0 = 8 * q + 4 * (L >> 5);
-        uint32_t d[4][kSlices];
+        const int ig0 = 16 * q + 8 * (L >> 5);
+        float m[kProds][8];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < 8; ++e) {
             const int ig = ig0 + e;
             double2 v = make_double2(0.0, 0.0);
             if (row < nb && ig < C) v = aqsn[(size_t)row * C + ig];
-            split3((float)v.x, (float)v.y, d[e]);
+            m[0][e] = (float)(v.x + v.y);
+            m[1][e] = (float)v.x;
+            m[2][e] = (float)v.y;
         }
 #pragma unroll
-        for (int s = 0; s < kSlices; ++s)
-            As[(((size_t)q * RF + r) * kSlices + s) * 64 + L] = make_uint4(d[0][s], d[1][s], d[2][s], d[3][s]);
+        for (int pr = 0; pr < kProds; ++pr) {
+            uint32_t d[4][kSlices];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) split3(m[pr][2 * e], m[pr][2 * e + 1], d[e]);
+#pragma unroll
+            for (int sl = 0; sl < kSlices; ++sl)
+                As[((((size_t)q * RF + r) * kProds + pr) * kSlices + sl) * 64 + L] =
+                    make_uint4(d[0][sl], d[1][sl], d[2][sl], d[3][sl]);
+        }
     }
 }
 
@@ -148,63 +164,75 @@ __global__ __launch_bounds__(256, 1) void ff_split_gemm_kernel(SplitArgs a)
     // two fragments and its results are dropped
     const int r0 = min(8 * mt + 2 * w, a.RF - 2);
 
-    // X elements this thread loads: columns 16w + 2j + (L >> 5) of the tile
-    // (j = 0..7, clamped to the last column), ig = stage*32 + (L & 31)
-    // clamped to the last ig (where the split aqsn is zero).  Addresses are
-    // 32-bit offsets from a per-wave base.
-    const int kk = L & 31;
+    // X elements this thread loads: ig pair ip = L & 15 (ig = stage*32 + 2ip,
+    // +1, clamped to the last ig, where the split aqsn is zero) of columns
+    // 16w + 4j + (L >> 4) of the tile (j = 0..3, clamped to the last
+    // column).  Addresses are 32-bit offsets from a per-wave base.
+    const int ip = L & 15;
     const int wcol = min(gt * kTileN + 16 * w, a.G - 1);
     const double2 *xbase = a.X + ((size_t)f * a.G + wcol) * a.C;
-    uint32_t xoff[8];
+    uint32_t xoff[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int col = min(gt * kTileN + 16 * w + 2 * j + (L >> 5), a.G - 1);
+    for (int j = 0; j < 4; ++j) {
+        const int col = min(gt * kTileN + 16 * w + 4 * j + (L >> 4), a.G - 1);
         xoff[j] = (uint32_t)(col - wcol) * (uint32_t)a.C;
     }
-    auto load_x = [&](int s, int j) -> double2 {
-        const uint32_t ig = (uint32_t)min((s0 + s) * kStageIg + kk, a.C - 1);
-        return xbase[xoff[j] + ig];
+    struct XPair { double2 v0, v1; };
+    auto load_x = [&](int s, int j) -> XPair {
+        const uint32_t ig = (uint32_t)((s0 + s) * kStageIg + 2 * ip);
+        const uint32_t last = (uint32_t)(a.C - 1);
+        return XPair{xbase[xoff[j] + min(ig, last)], xbase[xoff[j] + min(ig + 1, last)]};
     };
-    // split one X element into the B1 and B2 slices of LDS stage `buf`
-    auto store_x = [&](int buf, int j, double2 v) {
-        uint32_t d[kSlices];
-        split3((float)v.x, (float)v.y, d);
-        uint32_t *p = lds + buf * kBufDw + (16 * w + 2 * j + (L >> 5)) * kRowDw + kk;
+    // split one pair of X elements into the slices of the three B matrices
+    // (xr, xi - xr, xr + xi) of LDS stage `buf`
+    auto store_x = [&](int buf, int j, const XPair &x) {
+        float m[kProds][2];
+        m[0][0] = (float)x.v0.x;
+        m[0][1] = (float)x.v1.x;
+        m[1][0] = (float)(x.v0.y - x.v0.x);
+        m[1][1] = (float)(x.v1.y - x.v1.x);
+        m[2][0] = (float)(x.v0.x + x.v0.y);
+        m[2][1] = (float)(x.v1.x + x.v1.y);
+        uint32_t *p = lds + buf * kBufDw + (16 * w + 4 * j + (L >> 4)) * kRowDw + ip;
 #pragma unroll
-        for (int s = 0; s < kSlices; ++s) {
-            p[s * kSliceDw] = d[s] ^ 0x80000000u;                                  // B1: (Re x, -Im x)
-            p[(kSlices + s) * kSliceDw] = __builtin_rotateright32(d[s], 16);      // B2: (Im x, Re x)
+        for (int pr = 0; pr < kProds; ++pr) {
+            uint32_t d[kSlices];
+            split3(m[pr][0], m[pr][1], d);
+#pragma unroll
+            for (int sl = 0; sl < kSlices; ++sl) p[(pr * kSlices + sl) * kSliceDw] = d[sl];
         }
     };
-    const uint4 *As = a.As + (size_t)r0 * kSlices * 64 + L;
-    const size_t qstride = (size_t)a.RF * kSlices * 64;   // uint4 per k-step
-    auto load_a = [&](int s, int q, int i, int sl) -> uint4 {
-        return As[((size_t)(s0 + s) * kSteps + q) * qstride + (i * kSlices + sl) * 64];
+    const uint4 *As = a.As + (size_t)r0 * kProds * kSlices * 64 + L;
+    const size_t qstride = (size_t)a.RF * kProds * kSlices * 64;   // uint4 per k-step
+    auto load_a = [&](int s, int q, int pr, int i, int sl) -> uint4 {
+        return As[((size_t)(s0 + s) * kSteps + q) * qstride + ((i * kProds + pr) * kSlices + sl) * 64];
     };
 
-    f32x16 acc[2][2][2];
+    f32x16 acc[2][2][kProds];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
+            for (int pr = 0; pr < kProds; ++pr)
 #pragma unroll
-                for (int e = 0; e < 16; ++e) acc[i][j][c][e] = 0.f;
+                for (int e = 0; e < 16; ++e) acc[i][j][pr][e] = 0.f;
 
-    // A fragments of one stage (one set per k-step) and X of the next stage
-    uint4 av[kSteps][2][kSlices];
-    double2 xv[8];
+    // A fragments of one stage and X of the next stage
+    uint4 av[kSteps][kProds][2][kSlices];
+    XPair xv[4];
 #pragma unroll
     for (int q = 0; q < kSteps; ++q)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int pr = 0; pr < kProds; ++pr)
 #pragma unroll
-            for (int sl = 0; sl < kSlices; ++sl) av[q][i][sl] = load_a(0, q, i, sl);
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) store_x(0, j, load_x(0, j));
+                for (int sl = 0; sl < kSlices; ++sl) av[q][pr][i][sl] = load_a(0, q, pr, i, sl);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = load_x(min(1, nst - 1), j);
+    for (int j = 0; j < 4; ++j) store_x(0, j, load_x(0, j));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xv[j] = load_x(min(1, nst - 1), j);
     __syncthreads();
 
     for (int s = 0; s < nst; ++s) {
@@ -212,45 +240,40 @@ __global__ __launch_bounds__(256, 1) void ff_split_gemm_kernel(SplitArgs a)
         const int sn = min(s + 1, nst - 1), sx = min(s + 2, nst - 1);
 #pragma unroll
         for (int q = 0; q < kSteps; ++q) {
-            uint4 b[2][2][kSlices];   // [B1, B2][column block][slice]
 #pragma unroll
-            for (int v = 0; v < 2; ++v)
+            for (int pr = 0; pr < kProds; ++pr) {
+                uint4 b[2][kSlices];   // [column block][slice] of B matrix pr
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
 #pragma unroll
                     for (int sl = 0; sl < kSlices; ++sl)
-                        b[v][j][sl] = *reinterpret_cast<const uint4 *>(buf + (v * kSlices + sl) * kSliceDw +
+                        b[j][sl] = *reinterpret_cast<const uint4 *>(buf + (pr * kSlices + sl) * kSliceDw +
                                                                     (32 * j + (L & 31)) * kRowDw + 8 * q +
                                                                     4 * (L >> 5));
-            // slice pairs (A, B) in six groups: (0,0) (0,1) (0,2) (1,0) (1,1) (2,0)
-            int g = 0;
+                // slice pairs (A, B): (0,0) (0,1) (0,2) (1,0) (1,1) (2,0)
 #pragma unroll
-            for (int sa = 0; sa < kSlices; ++sa) {
+                for (int sa = 0; sa < kSlices; ++sa)
 #pragma unroll
-                for (int sb = 0; sb < kSlices - sa; ++sb, ++g) {
+                    for (int sb = 0; sb < kSlices - sa; ++sb)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i)
+                        for (int i = 0; i < 2; ++i)
 #pragma unroll
-                        for (int j = 0; j < 2; ++j) {
-                            acc[i][j][0] = mfma(av[q][i][sa], b[0][j][sb], acc[i][j][0]);
-                            acc[i][j][1] = mfma(av[q][i][sa], b[1][j][sb], acc[i][j][1]);
-                        }
-                    // work between the groups: X of stage s+1 split into the
-                    // other LDS buffer (last read before the previous
-                    // barrier), each element reloaded with stage s+2 at once
-                    // (k-steps 1 and 2), and the A fragments of this k-step
-                    // reloaded with stage s+1 after their last use
-                    const int jx = (q == 1) ? g : (q == 2 && g < 2) ? 6 + g : -1;
-                    if (jx >= 0) {
-                        store_x((s + 1) & 1, jx, xv[jx]);
-                        xv[jx] = load_x(sx, jx);
-                    }
-                    if (sb == kSlices - sa - 1) {
-#pragma unroll
-                        for (int i = 0; i < 2; ++i) av[q][i][sa] = load_a(sn, q, i, sa);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
+                            for (int j = 0; j < 2; ++j)
+                                acc[i][j][pr] = mfma(av[q][pr][i][sa], b[j][sb], acc[i][j][pr]);
+                // work between the groups: X of stage s+1 split into the
+                // other LDS buffer (last read before the previous barrier),
+                // each pair reloaded with stage s+2 at once (groups 1..4),
+                // and this group's A fragments reloaded with stage s+1
+                const int g = q * kProds + pr;
+                if (g >= 1 && g <= 4) {
+                    store_x((s + 1) & 1, g - 1, xv[g - 1]);
+                    xv[g - 1] = load_x(sx, g - 1);
                 }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int sl = 0; sl < kSlices; ++sl) av[q][pr][i][sl] = load_a(sn, q, pr, i, sl);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();
@@ -269,7 +292,8 @@ __global__ __launch_bounds__(256, 1) void ff_split_gemm_kernel(SplitArgs a)
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const int row = 32 * i + 8 * (e >> 2) + 4 * (L >> 5) + (e & 3);
-                E[row * kEpiRow + (L & 31)] = make_float2(acc[i][j][0][e], acc[i][j][1][e]);
+                const double k1 = acc[i][j][0][e], k2 = acc[i][j][1][e], k3 = acc[i][j][2][e];
+                E[row * kEpiRow + (L & 31)] = make_float2((float)(k1 - k3), (float)(k1 + k2));
             }
         __syncthreads();
         const int c0 = gt * kTileN + 32 * j;
@@ -296,8 +320,9 @@ __global__ __launch_bounds__(256, 1) void ff_split_gemm_kernel(SplitArgs a)
 
 // stages per ig chunk for fk_options.config = 0 (default), 1, 2, 3: ig
 // chunks of 1024, 256, 512 and 1024.  At 256/4096/4096/32 on MI355X the
-// chunk of 1024 runs 5.4 ms against 5.7 ms for 512 (fewer workgroups, each
-// paying one prologue and one epilogue), with error 6.6e-7 against 4.9e-7.
+// chunk of 1024 runs 4.5 ms with error 6.8e-7 against FP64; 2048 runs 4.33
+// ms at 9.6e-7 and 4096 4.30 ms at 1.5e-6 (the FP32 sums grow with the
+// chunk), so the default stays at 1024.
 constexpr int kSplitNs[] = {32, 8, 16, 32};
 
 }  // namespace
@@ -314,7 +339,7 @@ void ff_split_plan(int nb, int C, int G, int nf, int config, Scratch &ws, FfSpli
     L.ngt = div_up(G, kTileN);
     L.total = L.nkc * L.nmt * L.ngt * nf;
     const size_t nq = (size_t)L.nstages * kSteps;
-    L.off_as = ws.plan(nq * L.RF * kSlices * 64 * sizeof(uint4));
+    L.off_as = ws.plan(nq * L.RF * kProds * kSlices * 64 * sizeof(uint4));
 }
 
 int ff_split_launch(const FfSplitLayout &L, Scratch &ws, const double2 *aqsn, const double2 *X, const double2 *cf,
