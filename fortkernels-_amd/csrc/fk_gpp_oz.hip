@@ -5,36 +5,40 @@
 // 59 (2012); on integer matrix units: Ootomo, Ozaki & Yokota, IJHPCA 2024);
 // everything else is FP64.
 //
-// Inputs are complex64 (fk_cfloat).  Each row of the coefficient matrix
-// c[n1][my_igp] = 0.5 vcoul[igp] conj(aqsm[n1][igp]) (FP64) and each column
-// of aqsn is scaled so that its largest |re|, |im| (M) maps to 127, and
-// split into four int8 slices:
+// Inputs are complex64 (fk_cfloat).  The complex product runs as three real
+// products (Gauss):
+//     P0 = cr ar,   P1 = ci ai,   P2 = (cr + ci)(ar + ai),
+//     re B = P0 - P1,   im B = P2 - P0 - P1,
+// over the matrices cr, ci, cr + ci of the coefficients c[n1][my_igp] =
+// 0.5 vcoul[igp] conj(aqsm[n1][igp]) (FP64) and ar, ai, ar + ai of aqsn.
+// Each row of each coefficient matrix and each column of each aqsn matrix
+// is scaled so that its largest magnitude (M) maps to 127 and split into
+// four int8 slices:
 //     x = M / 127 * (s0 + s1/254 + s2/254^2 + s3/254^3),  |s_k| <= 127,
-// 31 bits of the row's or column's range.  A complex product is a real GEMM
-// over an interleaved K' = 2 nb axis:
-//     re B = sum (cr, -ci) . (ar, ai),   im B = sum (ci, cr) . (ar, ai),
-// and the slice pairs (i, j) with i + j = L share the weight 254^-L, so five
-// int32 accumulators per component (L = 0..4; the pairs with i + j > 4 are
-// below 254^-5 of the leading term and dropped) hold the contraction
-// exactly.  The error against the FP64 evaluation on the same complex64
-// values is the slice truncation's, a few 1e-10 relative.
+// 31 bits of the row's or column's range.  The slice pairs (i, j) with
+// i + j = L share the weight 254^-L, so four int32 accumulators per product
+// (L = 0..3; the pairs with i + j > 3 are below 254^-4 of the leading term
+// and dropped) hold the contraction exactly: 3 x 10 slice pairs, 30 int8
+// MFMAs per 32 bands.  The error against the FP64 evaluation on the same
+// complex64 values is the slice truncation's, a few 1e-9 relative at most
+// (2.4e-9 at 512/2/32768/1638).
 //
 // Kernels (one call = four launches on the caller's stream):
-//   oz_prep_kernel       row / column maxima of the coefficients and aqsn,
-//                        their int8 slices, FP64 occupied rows
-//   gpp_oz_gemm_kernel   int8 MFMA tile, operands staged through LDS by
-//                        direct global->LDS loads in a five-deep ring;
+//   oz_prep_kernel       row / column maxima of the six matrices, their
+//                        int8 slices, FP64 occupied rows
+//   gpp_oz_gemm_kernel   int8 MFMA tile, operands staged through a
+//                        three-stage LDS ring by direct global->LDS loads;
 //                        B rebuilt in FP64 and stored
 //   gpp_oz_pole_kernel   per (my_igp, ig): occupied-band sum Bo in FP64, the
 //                        pole and SX factors in FP64, per-block partials
 //   gpp_reduce_kernel    fixed-order sum of the partials (shared with FP64)
 //
-// Slices are stored in MFMA-lane order: a 1 KB block per (slice, 32 rows,
-// 32 K') in which lane L's 16 bytes (row L%32, K' 16 (L/32) .. +15) sit at
-// byte 16 L, so one direct-to-LDS load moves a whole operand fragment of
-// the 32x32x32 int8 MFMA and the MFMA reads it back with one conflict-free
-// ds_read_b128 per lane.  (The 16x16x64 int8 MFMA runs at half the int8
-// rate on this part; the 32x32x32 form reaches it.)
+// Slices are stored in MFMA-lane order: a 1 KB block per (plane, 32 rows,
+// 32 bands) in which lane L's 16 bytes (row L%32, bands 16 (L/32) .. +15)
+// sit at byte 16 L, so one direct-to-LDS load moves a whole operand
+// fragment of the 32x32x32 int8 MFMA and the MFMA reads it back with one
+// conflict-free ds_read_b128 per lane.  (The 16x16x64 int8 MFMA runs at
+// half the int8 rate on this part; the 32x32x32 form reaches it.)
 #include "fk_complex.h"
 #include "fk_gpp_common.h"
 #include "fk_internal.h"
@@ -50,11 +54,10 @@ using namespace gpp;
 typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSlices = 4;
-constexpr int kLevels = 5;      // slice pairs with i + j < kLevels
-constexpr int kKStep = 64;      // K' per pipeline step = 32 bands = two MFMA k-halves
-constexpr int kKHalf = 32;      // K' per 32x32x32 int8 MFMA
-constexpr int kBandsPerStep = kKStep / 2;
-constexpr int kPlanesA = kSlices, kPlanesB = kSlices;
+constexpr int kProds = 3;       // P0 = cr ar, P1 = ci ai, P2 = (cr + ci)(ar + ai)
+constexpr int kLevels = 4;      // slice pairs with i + j < kLevels
+constexpr int kKStep = 32;      // bands per pipeline step = the K of one 32x32x32 int8 MFMA
+constexpr int kPlanes = kProds * kSlices;   // slice planes per operand
 constexpr int kFrag = 1024;     // bytes per lane-ordered fragment
 constexpr int kMaxBandsOz = 16000;   // keeps the level-3 int32 sums (4 pairs) below 2^31
 
@@ -84,21 +87,22 @@ __device__ __forceinline__ double2 coef(const float2 *aqsm, const double *vcoul,
 
 // ---------------------------------------------------------------------------
 // Slices of both operands in one launch.  A workgroup takes 4 coefficient
-// rows (the first ldm/4 workgroups) or 16 aqsn columns (the rest): it finds
-// their max(|re|, |im|) over all bands, then splits them band block by band
-// block (8 bands = the 16 bytes of one fragment lane); the second pass
-// finds the workgroup's slab in L2.  Thread t of an R-wide slab: row or
-// column R x + t%R, band blocks t/R, t/R + 256/R, ...  Coefficient slices
-// hold (cr, -ci) (the GEMM derives (ci, cr)); aqsn slices hold (ar, ai).
+// rows (the first ldm/4 workgroups) or 16 aqsn columns (the rest).  Each
+// row (column) feeds three real matrices, cr, ci, cr + ci (ar, ai, ar + ai),
+// each with its own scale: the workgroup finds the three maxima over all
+// bands, then splits the three values of each band into four slices, band
+// block by band block (16 bands = the 16 bytes of one fragment lane); the
+// second pass finds the workgroup's slab in L2.  Thread t of an R-wide
+// slab: row or column R x + t%R, band blocks t/R, t/R + 256/R, ...
 // ---------------------------------------------------------------------------
 struct PrepArgs {
     const float2 *aqsm, *aqsn;
     const double *vcoul;
     const int *igp;
-    int8_t *as, *bs;         // [4][ldm/32][Kp/32][1024], [4][ldn/32][Kp/32][1024]
-    double *ra, *rb;         // M / 127 of each row, column
+    int8_t *as, *bs;         // [12][ldm/32][nkb][1024], [12][ldn/32][nkb][1024]: plane = matrix * 4 + slice
+    double *ra, *rb;         // [3][ldm], [3][ldn]: M / 127 of each row, column of each matrix
     double2 *c_occ;          // [nv][ldm] coefficients of the occupied bands
-    int nb, nv, C, G, ldm, ldn, nkh, blocks_a;
+    int nb, nv, C, G, ldm, ldn, nkb, blocks_a;
 };
 
 // R rows (coefficients, IS_A) or columns (aqsn) per workgroup, 256 / R band
@@ -108,61 +112,70 @@ template <int R, bool IS_A>
 __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
 {
     constexpr int NG = 256 / R;
-    __shared__ double sm[NG][R];
+    __shared__ double sm[kProds][NG][R];
     const int tid = threadIdx.x, c = tid % R, grp = tid / R;
     const int idx = slab * R + c;
     const int g = IS_A ? (idx < a.G ? a.igp[idx] : -1) : 0;
     const bool in = IS_A ? g >= 0 : idx < a.C;
-    // element n1 of this row / column as (first, second) of its stored pair
+    // element n1 of this row (the coefficient c) or column (aqsn)
     auto value = [&](int n1) -> double2 {
         if (!in || n1 >= a.nb) return make_double2(0.0, 0.0);
         if constexpr (IS_A) {
-            const double2 cf = coef(a.aqsm, a.vcoul, a.C, n1, g);
-            return make_double2(cf.x, -cf.y);
+            return coef(a.aqsm, a.vcoul, a.C, n1, g);
         } else {
             const float2 v = a.aqsn[(size_t)n1 * a.C + idx];
             return make_double2((double)v.x, (double)v.y);
         }
     };
-    const int nblk = (a.nb + 7) / 8;
-    double mx = 0.0;
+    const int nblk = (a.nb + 15) / 16;
+    double mx[kProds] = {0.0, 0.0, 0.0};
     for (int j = grp; j < nblk; j += NG)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const double2 v = value(8 * j + b);
-            mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+        for (int b = 0; b < 16; ++b) {
+            const double2 v = value(16 * j + b);
+            mx[0] = fmax(mx[0], fabs(v.x));
+            mx[1] = fmax(mx[1], fabs(v.y));
+            mx[2] = fmax(mx[2], fabs(v.x + v.y));
         }
-    sm[grp][c] = mx;
+#pragma unroll
+    for (int pr = 0; pr < kProds; ++pr) sm[pr][grp][c] = mx[pr];
     __syncthreads();
-    mx = 0.0;
-    for (int k = 0; k < NG; ++k) mx = fmax(mx, sm[k][c]);
-    const double scale = mx > 0.0 ? 127.0 / mx : 0.0;
-    if (grp == 0) (IS_A ? a.ra : a.rb)[idx] = mx / 127.0;
+    const int ld = IS_A ? a.ldm : a.ldn;
+    double scale[kProds];
+#pragma unroll
+    for (int pr = 0; pr < kProds; ++pr) {
+        double m = 0.0;
+        for (int k = 0; k < NG; ++k) m = fmax(m, sm[pr][k][c]);
+        scale[pr] = m > 0.0 ? 127.0 / m : 0.0;
+        if (grp == 0) (IS_A ? a.ra : a.rb)[(size_t)pr * ld + idx] = m / 127.0;
+    }
 
     int8_t *base = IS_A ? a.as : a.bs;
-    const size_t blk = idx >> 5, nblks = (size_t)(IS_A ? a.ldm : a.ldn) / 32;
-    for (int j = grp; j < a.nkh * 2; j += NG) {   // band blocks of the padded K'
-        int8_t p[kSlices][16];
+    const size_t blk = idx >> 5, nblks = (size_t)ld / 32;
+    for (int j = grp; j < a.nkb * 2; j += NG) {   // 16-band blocks of the padded K
+        int8_t q[kProds][kSlices][16];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const int n1 = 8 * j + b;
+        for (int b = 0; b < 16; ++b) {
+            const int n1 = 16 * j + b;
             const double2 v = value(n1);
-            if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = make_double2(v.x, -v.y);
-            int8_t s0[kSlices], s1[kSlices];
-            split4(v.x, scale, s0);
-            split4(v.y, scale, s1);
+            if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = v;
+            const double x[kProds] = {v.x, v.y, v.x + v.y};
 #pragma unroll
-            for (int s = 0; s < kSlices; ++s) {
-                p[s][2 * b] = s0[s];
-                p[s][2 * b + 1] = s1[s];
+            for (int pr = 0; pr < kProds; ++pr) {
+                int8_t sl[kSlices];
+                split4(x[pr], scale[pr], sl);
+#pragma unroll
+                for (int k = 0; k < kSlices; ++k) q[pr][k][b] = sl[k];
             }
         }
-        const int lane = (idx & 31) + 32 * (j & 1), kh = j >> 1;
+        const int lane = (idx & 31) + 32 * (j & 1), kb = j >> 1;
 #pragma unroll
-        for (int pl = 0; pl < kSlices; ++pl) {
-            int8_t *dst = base + ((pl * nblks + blk) * a.nkh + kh) * kFrag + 16 * lane;
-            *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(p[pl]);
-        }
+        for (int pr = 0; pr < kProds; ++pr)
+#pragma unroll
+            for (int k = 0; k < kSlices; ++k) {
+                int8_t *dst = base + (((size_t)(pr * kSlices + k) * nblks + blk) * a.nkb + kb) * kFrag + 16 * lane;
+                *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(q[pr][k]);
+            }
     }
 }
 
@@ -182,43 +195,48 @@ __global__ __launch_bounds__(256) void oz_prep_kernel(PrepArgs a)
 // ---------------------------------------------------------------------------
 // int8 MFMA GEMM.  64x64 workgroup tile, 2x2 waves of one 32x32 block
 // each.  32x32x32 int8 MFMA: lane l supplies 16 bytes of row l%32 of A and
-// of column l%32 of B at the same K' positions; accumulator register j of
-// lane l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32.
+// of column l%32 of B at the same 16 bands; accumulator register j of lane
+// l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32.
 //
-// LDS ring of kStages k-steps; a stage holds the workgroup's 16 A fragments
-// ([slice][row block][k-half]) and 16 B fragments ([slice][column
-// block][k-half]), 32 KB.  Each wave moves 8 fragments per step with direct
-// global->LDS loads, issued four steps before the step that uses them.
-// A step is two halves (32 K' each).  A half runs its 26 MFMAs from one
-// register set and, between them at fixed positions, reads the next
-// half's 8 fragments from LDS into the other set and derives their (ci, cr)
-// pairs from the stored (cr, -ci) ones, so LDS traffic and the byte
-// shuffles overlap the MFMAs.  Between the halves of step s the wave waits
-// for its loads of step s+1 (later steps may stay in flight) and passes a
-// barrier that covers every wave's loads; the second half then issues the
-// loads of step s+4, spread over its MFMAs, into the stage step s-1 used.  Scheduling barriers hold the order; two
-// MFMAs into one accumulator are at least five apart.
+// The complex product as three real products (Gauss):
+//     P0 = cr ar,   P1 = ci ai,   P2 = (cr + ci)(ar + ai),
+//     re B = P0 - P1,   im B = P2 - P0 - P1,
+// each matrix with its own row (column) scales.  Each product runs the ten
+// slice pairs of levels 0..3: 30 MFMAs per 32 bands.  (The complex product
+// as two real products over an interleaved K' = 2 nb with one scale per row
+// and column needs all 13 pairs of levels 0..4 for the same accuracy: 52
+// MFMAs.)
+//
+// LDS ring of kStages steps of 32 bands; a stage holds the workgroup's 24 A
+// fragments ([matrix][slice][row block]) and 24 B fragments
+// ([matrix][slice][column block]), 48 KB.  Each wave moves 12 fragments per
+// step with direct global->LDS loads, issued over the first half of step s
+// for step s+2, into the stage step s-1 used.  At the middle of step s the
+// wave waits for its loads of step s+1 and passes a barrier that covers
+// every wave's loads; the second half reads step s+1's fragments into the
+// other register set.  Scheduling barriers hold the order; two MFMAs into
+// one accumulator are at least nine apart.
 // ---------------------------------------------------------------------------
-constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 5, kAhead = kStages - 1;
-constexpr int kFragsA = kPlanesA * (kBM / 32) * 2, kFragsB = kPlanesB * (kBN / 32) * 2;
-constexpr int kFragsPerStage = kFragsA + kFragsB;          // 32
-constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 8
+constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 3;
+constexpr int kFragsA = kPlanes * (kBM / 32), kFragsB = kPlanes * (kBN / 32);   // 24, 24
+constexpr int kFragsPerStage = kFragsA + kFragsB;          // 48
+constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 12
 
-// MFMA order of one half: (slice of A, slice of B); both variants follow
-// each entry.  Levels si + sj: 3 4 2 3 4 1 3 2 4 3 0 2 1.
+// slice pairs of a step, each followed by its three products; levels
+// si + sj: 3 2 1 3 2 1 3 2 0 3
 struct PairStep { int si, sj; };
-constexpr int kPairs = 13;
-constexpr int kMfmasHalf = 2 * kPairs;                     // 26
-constexpr PairStep kOrder[kPairs] = {{0, 3}, {1, 3}, {0, 2}, {1, 2}, {2, 2}, {0, 1}, {2, 1},
-                                     {1, 1}, {3, 1}, {3, 0}, {0, 0}, {2, 0}, {1, 0}};
-// positions in a half's MFMA stream: the 8 LDS reads follow MFMAs 0..7, the
-// 16 dword shuffles follow MFMAs kRotFirst .. kRotFirst + 15
-constexpr int kRotFirst = 9;
+constexpr int kPairs = 10;
+constexpr PairStep kOrder[kPairs] = {{0, 3}, {0, 2}, {0, 1}, {1, 2}, {1, 1},
+                                     {1, 0}, {2, 1}, {2, 0}, {0, 0}, {3, 0}};
+constexpr int kMfmasStep = kPairs * kProds;   // 30
+constexpr int kHalf = 15;                     // MFMAs before the mid-step wait
+static_assert(kLoadsPerWave <= kHalf && 2 * (kMfmasStep - kHalf) >= 2 * kPlanes, "schedule slots");
 
 struct GemmArgs {
     const int8_t *as, *bs;
-    double2 *bmat;           // [ldm][ldn] B / (ra[m] rb[ig]) in FP64
-    int ldm, ldn, nkh, tiles_m, tiles_n, group_m;
+    const double *ra, *rb;   // [3][ldm], [3][ldn]: M / 127 of each row, column of each matrix
+    double2 *bmat;           // [ldm][ldn] B in FP64
+    int ldm, ldn, nkb, tiles_m, tiles_n, group_m;
 };
 
 typedef int i16v __attribute__((ext_vector_type(16)));
@@ -226,18 +244,6 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ i16v mfma_i8(i4 a, i4 b, i16v c)
 {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
-}
-
-// (cr, -ci) byte pairs -> (ci, cr) in one dword: negate the high byte of
-// each 16-bit half (two's complement within the half; the carry out of bit
-// 15 is dropped by the 16-bit lane), then swap the two bytes of each half.
-__device__ __forceinline__ int rotate_pair_dword(int x)
-{
-    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-    const unsigned d = (unsigned)x ^ 0xFF00FF00u;
-    const u16x2 h = __builtin_bit_cast(u16x2, d) + u16x2{0x0100, 0x0100};
-    const unsigned n = __builtin_bit_cast(unsigned, h);
-    return (int)__builtin_amdgcn_perm(n, n, 0x02030001u);
 }
 
 // Workgroup b -> tile.  Each XCD runs a contiguous range of the logical
@@ -257,21 +263,29 @@ __device__ __forceinline__ void oz_tile(int b, int tiles_m, int tiles_n, int gro
     tn = u / gm;
 }
 
-// wait until at most n of this wave's fragment loads are outstanding, then
-// pass the workgroup barrier (n = 8 x the steps issued after the awaited one)
+// wait until at most n (12 or 0) of this wave's fragment loads are
+// outstanding and its LDS reads are done, then pass the workgroup barrier
 __device__ __forceinline__ void wait_loads_barrier(int n)
 {
-    switch (n) {
-    case 24: asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory"); break;
-    case 16: asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory"); break;
-    case 8: asm volatile("s_waitcnt vmcnt(8)\n\ts_barrier" ::: "memory"); break;
-    default: asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); break;
-    }
+    if (n == 12) asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-struct HalfFrags {
-    i4 a[kSlices][2];        // [slice][variant]
-    i4 b[kSlices];           // [slice]
+// One direct global->LDS load of 16 bytes per lane: lane l's bytes land at
+// LDS byte address lds + 16 l.  Written as inline asm so that the compiler's
+// wait insertion, which cannot tell which LDS these loads write, does not
+// make every LDS read wait for every load in flight; the kernel orders its
+// loads and reads itself with explicit vmcnt waits and barriers.  The asm
+// has no register outputs.
+__device__ __forceinline__ void lds_load16(const int8_t *g, const int8_t *lds)
+{
+    const unsigned la = (unsigned)(uintptr_t)lds;   // the LDS offset: low half of the shared-aperture address
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(la) : "memory", "m0");
+}
+
+struct StepFrags {
+    i4 a[kProds][kSlices];   // this wave's row block
+    i4 b[kProds][kSlices];   // this wave's column block
 };
 
 // one 64x64 tile (workgroup b of the problem's launch)
@@ -279,80 +293,80 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
     oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
-    const int nks = a.nkh / 2;
+    const int nks = a.nkb;
 
-    // global source of this wave's fragment loads: stage fragment f is A
-    // (slice f/4, row block 2 tm + (f/2)%2, k-half f%2) for f < 16, else B
-    // (slice (f-16)/4, column block 2 tn + ((f-16)/2)%2, k-half f%2);
-    // consecutive k-steps are two fragments apart
-    const size_t nrb = (size_t)a.ldm / 32, ncb = (size_t)a.ldn / 32;
-    const int8_t *src[kLoadsPerWave];
-#pragma unroll
-    for (int k = 0; k < kLoadsPerWave; ++k) {
-        const int f = wave * kLoadsPerWave + k;
-        const int8_t *base;
-        if (f < kFragsA) {
-            const int pl = f >> 2, rb = 2 * tm + ((f >> 1) & 1), kh = f & 1;
-            base = a.as + (((size_t)pl * nrb + rb) * a.nkh + kh) * kFrag;
-        } else {
-            const int g = f - kFragsA, pl = g >> 2, cb = 2 * tn + ((g >> 1) & 1), kh = g & 1;
-            base = a.bs + (((size_t)pl * ncb + cb) * a.nkh + kh) * kFrag;
-        }
-        src[k] = base + 16 * lane;
-    }
+    // this wave's 12 fragment loads per step: stage fragment f = 12 wave + k
+    // is A (plane f/2, row block 2 tm + f%2) for f < 24, else B (plane
+    // (f-24)/2, column block 2 tn + f%2); consecutive steps are one
+    // fragment apart.  Waves 0 and 1 move A, waves 2 and 3 B.
+    const bool side_a = wave < 2;
+    const size_t nblk = (size_t)(side_a ? a.ldm : a.ldn) / 32;
+    const int t2 = 2 * (side_a ? tm : tn);
+    const int8_t *vbase = (side_a ? a.as : a.bs) + 16 * lane;
+    auto src_off = [&](int k) -> size_t {   // uniform
+        const int f = (wave & 1) * kLoadsPerWave + k;
+        return (((size_t)(f >> 1) * nblk + t2 + (f & 1)) * a.nkb) * kFrag;
+    };
     auto issue_one = [&](int s, int k) {
-        int8_t *dst = &lds[s % kStages][wave * kLoadsPerWave][0];
-        __builtin_amdgcn_global_load_lds(src[k] + (size_t)s * 2 * kFrag, dst + k * kFrag, 16, 0, 0);
+        int8_t *dst = &lds[s % kStages][wave * kLoadsPerWave + k][0];
+        lds_load16(vbase + src_off(k) + (size_t)s * kFrag, dst);
     };
     auto issue = [&](int s) {
 #pragma unroll
         for (int k = 0; k < kLoadsPerWave; ++k) issue_one(s, k);
     };
-    // LDS read #k (0..7) of a half's fragments (stage st, k-half kh): B slice
-    // 3 - k/2 for even k, A slice k/2 for odd k (the order the MFMAs first
-    // need them)
-    auto read_frag = [&](const int8_t *st, int kh, HalfFrags &f, int k) {
-        const int sl = k >> 1;
-        if ((k & 1) == 0)
-            f.b[3 - sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + (3 - sl) * 4 + wn * 2 + kh) * kFrag + 16 * lane);
+    // LDS read #r (0..23) of a step's fragments: A (matrix r/8, slice
+    // (r/2)%4) for even r, B for odd r
+    auto read_frag = [&](const int8_t *st, StepFrags &f, int r) {
+        const int pr = r >> 3, sl = (r >> 1) & 3;
+        if ((r & 1) == 0)
+            f.a[pr][sl] = *reinterpret_cast<const i4 *>(st + (pr * 8 + sl * 2 + wm) * kFrag + 16 * lane);
         else
-            f.a[sl][0] = *reinterpret_cast<const i4 *>(st + (sl * 4 + wm * 2 + kh) * kFrag + 16 * lane);
+            f.b[pr][sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + pr * 8 + sl * 2 + wn) * kFrag + 16 * lane);
     };
-    // shuffle #k (0..15): dword k%4 of A slice k/4
-    auto rotate = [&](HalfFrags &f, int k) {
-        f.a[k >> 2][1][k & 3] = rotate_pair_dword(f.a[k >> 2][0][k & 3]);
-    };
+    auto stage = [&](int s) -> int8_t * { return &lds[s % kStages][0][0]; };
 
-    i16v acc[kLevels][2];
+    i16v acc[kProds][kLevels];
 #pragma unroll
-    for (int L = 0; L < kLevels; ++L)
+    for (int pr = 0; pr < kProds; ++pr)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[L][v] = i16v{};
+        for (int L = 0; L < kLevels; ++L) acc[pr][L] = i16v{};
 
-    // one half: MFMAs from `cur`; with NEXT, the following half's fragments
-    // (stage st, k-half kh) into `nxt`; with LOADS, the global->LDS loads of
-    // step `ls`, one after every other MFMA
-    auto half = [&](auto next_t, auto loads_t, const HalfFrags &cur, HalfFrags &nxt, const int8_t *st, int kh,
-                    int ls) {
+    // one step: MFMAs from `cur`; with LOADS, the loads of step s+2 over
+    // the first half; with NEXT, the mid-step wait for step s+1 and its
+    // fragments read into `nxt` over the second half
+    // The source (gsrc), the stage being loaded (wr) and the stage being
+    // read (rd) are restrict-qualified: after inlining, their accesses carry
+    // alias scopes, from which the compiler sees that the LDS reads of rd do
+    // not depend on the direct loads into wr issued in the same step, so it
+    // does not make the reads wait for those loads; the explicit wait above
+    // the reads covers the loads of step s+1.
+    auto step = [&](auto next_t, auto loads_t, const StepFrags &cur, StepFrags &nxt, int s,
+                    const int8_t *__restrict__ gsrc, int8_t *__restrict__ wr, const int8_t *__restrict__ rd) {
         constexpr bool NEXT = decltype(next_t)::value;
         constexpr bool LOADS = decltype(loads_t)::value;
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int o = 0; o < kMfmasHalf; ++o) {
-            const PairStep ps = kOrder[o >> 1];
-            const int v = o & 1;
-            acc[ps.si + ps.sj][v] = mfma_i8(cur.a[ps.si][v], cur.b[ps.sj], acc[ps.si + ps.sj][v]);
+        for (int o = 0; o < kMfmasStep; ++o) {
+            const PairStep ps = kOrder[o / kProds];
+            const int pr = o % kProds;
+            acc[pr][ps.si + ps.sj] = mfma_i8(cur.a[pr][ps.si], cur.b[pr][ps.sj], acc[pr][ps.si + ps.sj]);
             __builtin_amdgcn_sched_barrier(0);
-            if constexpr (NEXT) {
-                if (o < 8) read_frag(st, kh, nxt, o);
-                if (o >= kRotFirst && o < kRotFirst + 16) rotate(nxt, o - kRotFirst);
-            }
             if constexpr (LOADS) {
-                if ((o & 1) && o < 2 * kLoadsPerWave) issue_one(ls, o >> 1);
+                if (o < kLoadsPerWave)
+                    lds_load16(gsrc + src_off(o), wr + (wave * kLoadsPerWave + o) * kFrag);
+            }
+            if constexpr (NEXT) {
+                if (o == kHalf - 1) wait_loads_barrier(LOADS ? 12 : 0);
+                if (o >= kHalf && o < kHalf + kPlanes) {
+                    read_frag(rd, nxt, 2 * (o - kHalf));
+                    read_frag(rd, nxt, 2 * (o - kHalf) + 1);
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -360,47 +374,55 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
     using yes = std::integral_constant<bool, true>;
     using no = std::integral_constant<bool, false>;
 
-    HalfFrags h0, h1;
-#pragma unroll
-    for (int s = 0; s < kAhead; ++s)
-        if (s < nks) issue(s);
-    wait_loads_barrier(8 * min(kAhead - 1, nks - 1));
-#pragma unroll
-    for (int k = 0; k < 8; ++k) read_frag(&lds[0][0][0], 0, h0, k);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) rotate(h0, k);
-    auto stage = [&](int s) { return &lds[s % kStages][0][0]; };
-    int s = 0;
-    // steady state: step s+1 is awaited with steps s+2, s+3 in flight, and
-    // step s+4 exists
-    for (; s + kAhead < nks; ++s) {
-        half(yes{}, no{}, h0, h1, stage(s), 1, 0);                    // (s, 0); reads (s, 1)
-        asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
-        half(yes{}, yes{}, h1, h0, stage(s + 1), 0, s + kAhead);      // (s, 1); reads (s+1, 0)
-    }
-    // the last steps: fewer loads in flight, none issued
-    for (; s + 1 < nks; ++s) {
-        half(yes{}, no{}, h0, h1, stage(s), 1, 0);
-        wait_loads_barrier(8 * min(2, nks - 2 - s));
-        half(yes{}, no{}, h1, h0, stage(s + 1), 0, 0);
-    }
-    half(yes{}, no{}, h0, h1, stage(s), 1, 0);
-    half(no{}, no{}, h1, h0, stage(s), 0, 0);
-
-    // B = sum_L 254^-L acc_L in FP64 (int32 -> FP64 is exact); the row and
-    // column scales are applied where B is used
-    const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0, 1.0 / 4162314256.0};
+    // the step count is even and at least 2 (the plan pads the bands to a
+    // multiple of 64), so the steps pair up with the two register sets
+    StepFrags h0, h1;
+    issue(0);
+    issue(1);
+    // the tile's row and column scales, loaded now so that the epilogue
+    // does not wait on them; the row scales go through LDS
     const int col = tn * kBN + wn * 32 + (lane & 31);
+    const double rav = tid < kProds * kBM ? a.ra[(size_t)(tid / kBM) * a.ldm + tm * kBM + tid % kBM] : 0.0;
+    double rbc[kProds];
+#pragma unroll
+    for (int pr = 0; pr < kProds; ++pr) rbc[pr] = a.rb[(size_t)pr * a.ldn + col];
+    wait_loads_barrier(12);
+#pragma unroll
+    for (int r = 0; r < 2 * kPlanes; ++r) read_frag(stage(0), h0, r);
+    int s = 0;
+    // steady state: every step has a next step and loads to issue
+    auto src = [&](int s) { return vbase + (size_t)s * kFrag; };
+    for (; s + 2 < nks; s += 2) {
+        step(yes{}, yes{}, h0, h1, s, src(s + 2), stage(s + 2), stage(s + 1));
+        step(yes{}, yes{}, h1, h0, s + 1, src(s + 3), stage(s + 3), stage(s + 2));
+    }
+    // the last two steps
+    step(yes{}, no{}, h0, h1, s, src(s), stage(s + 2), stage(s + 1));
+    step(no{}, no{}, h1, h0, s + 1, src(s), stage(s + 3), stage(s + 2));
+
+    // B in FP64: P = (sum_L 254^-L acc_L) (M_row / 127) (M_col / 127) for
+    // each product (int32 -> FP64 is exact), then re = P0 - P1 and
+    // im = P2 - P0 - P1
+    // B is written once and read once by the pole kernel: streaming
+    // (nontemporal) stores keep it from displacing the operand slices in L2
+    __shared__ double sra[kProds][kBM];
+    if (tid < kProds * kBM) sra[tid / kBM][tid % kBM] = rav;
+    __syncthreads();
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const int row = tm * kBM + wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        double bre = 0.0, bim = 0.0;
+        const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        double P[kProds];
 #pragma unroll
-        for (int L = kLevels - 1; L >= 0; --L) {
-            bre += W[L] * (double)acc[L][0][j];
-            bim += W[L] * (double)acc[L][1][j];
+        for (int pr = 0; pr < kProds; ++pr) {
+            double v = 0.0;
+#pragma unroll
+            for (int L = kLevels - 1; L >= 0; --L) v += W[L] * (double)acc[pr][L][j];
+            P[pr] = v * sra[pr][r] * rbc[pr];
         }
-        a.bmat[(size_t)row * a.ldn + col] = make_double2(bre, bim);
+        const d2v bv = {P[0] - P[1], P[2] - P[0] - P[1]};
+        __builtin_nontemporal_store(bv, reinterpret_cast<d2v *>(&a.bmat[(size_t)(tm * kBM + r) * a.ldn + col]));
     }
 }
 
@@ -431,8 +453,7 @@ constexpr int kPoleCols = 256 * kPoleK;       // columns per workgroup
 constexpr int kOccLds = 64;                   // occupied coefficients staged in LDS
 
 struct PoleArgs {
-    const double2 *bmat;     // [ldm][ldn] B / (ra[m] rb[ig])
-    const double *ra, *rb;   // row and column scales of the slices
+    const double2 *bmat;     // [ldm][ldn] B
     const double2 *c_occ;    // [nv][ldm]
     const float2 *aqsn, *I_eps, *wtilde;
     const int *igp;
@@ -586,18 +607,15 @@ __device__ __forceinline__ void oz_pole_elems(const PoleArgs &a, const PoleLds &
                                               int c0, int c_hi, PoleElems &E, double2 &sachs)
 {
     const int tid = threadIdx.x;
-    const double ram = a.ra[m];
     const double2 *brow = a.bmat + (size_t)m * a.ldn;
     const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
     bool live[kPoleK];
-    double2 Bu[kPoleK], wt[kPoleK], ie[kPoleK], Bo[kPoleK];
-    double rbv[kPoleK];
+    double2 B[kPoleK], wt[kPoleK], ie[kPoleK], Bo[kPoleK];
 #pragma unroll
     for (int e = 0; e < kPoleK; ++e) {
         const int ig = c0 + e * 256 + tid;
         live[e] = ok && ig < c_hi;
-        Bu[e] = live[e] ? brow[ig] : make_double2(0.0, 0.0);
-        rbv[e] = live[e] ? a.rb[ig] : 0.0;
+        B[e] = live[e] ? brow[ig] : make_double2(0.0, 0.0);
         wt[e] = live[e] ? widen(wrow[ig]) : make_double2(0.0, 1.0);
         ie[e] = live[e] ? widen(erow[ig]) : make_double2(0.0, 0.0);
         Bo[e] = make_double2(0.0, 0.0);
@@ -616,11 +634,9 @@ __device__ __forceinline__ void oz_pole_elems(const PoleArgs &a, const PoleLds &
     }
 #pragma unroll
     for (int e = 0; e < kPoleK; ++e) {
-        const double sc = ram * rbv[e];
-        const double2 B = make_double2(Bu[e].x * sc, Bu[e].y * sc);
         const double2 Om = cmul(cmul(wt[e], wt[e]), ie[e]);
         E.wt[e] = wt[e];
-        E.Bie[e] = cmul(B, ie[e]);
+        E.Bie[e] = cmul(B[e], ie[e]);
         E.BoOm[e] = cmul(Bo[e], Om);
         E.Boie[e] = cmul(Bo[e], ie[e]);
         // static-CH remainder -sum B I_eps (gppkernel.deslippe.09.30.2014.f90:192-230)
@@ -647,14 +663,11 @@ __device__ __forceinline__ void oz_pole_stream(const PoleArgs &a, LDS &L, int b,
     for (int q = 0; q < W; ++q) { sach[q] = make_double2(0.0, 0.0); sasx[q] = make_double2(0.0, 0.0); }
     const int tid = threadIdx.x;
     if (ok) {
-        const double ram = a.ra[m];
         const double2 *brow = a.bmat + (size_t)m * a.ldn;
         const float2 *wrow = a.wtilde + (size_t)m * a.C, *erow = a.I_eps + (size_t)m * a.C;
         const int nvl = min(a.nv, kOccLds);
         for (int ig = c_lo + tid; ig < c_hi; ig += 256) {
-            const double2 Bu = brow[ig];
-            const double sc = ram * a.rb[ig];
-            const double2 B = make_double2(Bu.x * sc, Bu.y * sc);
+            const double2 B = brow[ig];
             const double2 wt = widen(wrow[ig]), ie = widen(erow[ig]);
             double2 Bo = make_double2(0.0, 0.0);
             for (int n1 = 0; n1 < nvl; ++n1) Bo = cadd(Bo, cmul(L.occ[n1], widen(a.aqsn[(size_t)n1 * a.C + ig])));
@@ -846,14 +859,14 @@ OzBlocks gpp_oz_batch_record(const fk_gpp_problem *p, const fk_options *o, const
     pr.rb = static_cast<double *>(ws.at(L.off_eb));
     pr.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
     pr.nb = p->number_bands; pr.nv = p->nvband; pr.C = p->ncouls; pr.G = p->ngpown;
-    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkh = L.Kp / kKHalf; pr.blocks_a = L.ldm / kPrepRowsA;
+    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkb = L.Kb / kKStep; pr.blocks_a = L.ldm / kPrepRowsA;
     GemmArgs &ga = r.ga;
-    ga.as = pr.as; ga.bs = pr.bs;
+    ga.as = pr.as; ga.bs = pr.bs; ga.ra = pr.ra; ga.rb = pr.rb;
     ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
-    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = pr.nkh;
+    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkb = pr.nkb;
     ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
     PoleArgs &pa = r.pa;
-    pa.bmat = ga.bmat; pa.ra = pr.ra; pa.rb = pr.rb; pa.c_occ = pr.c_occ; pa.aqsn = pr.aqsn;
+    pa.bmat = ga.bmat; pa.c_occ = pr.c_occ; pa.aqsn = pr.aqsn;
     pa.I_eps = static_cast<const float2 *>(p->I_eps_array); pa.wtilde = static_cast<const float2 *>(p->wtilde_array);
     pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
@@ -890,13 +903,13 @@ void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L)
     const int nb = std::max(p->number_bands, 1);
     L.ldm = div_up(p->ngpown, kBM) * kBM;
     L.ldn = div_up(p->ncouls, kBN) * kBN;
-    L.Kp = div_up(2 * nb, kKStep) * kKStep;
+    L.Kb = div_up(nb, 2 * kKStep) * 2 * kKStep;   // an even number of GEMM steps
     L.tiles_m = L.ldm / kBM;
     L.tiles_n = L.ldn / kBN;
-    L.off_as = ws.plan((size_t)kPlanesA * L.ldm * L.Kp);
-    L.off_bs = ws.plan((size_t)kPlanesB * L.ldn * L.Kp);
-    L.off_ea = ws.plan(sizeof(double) * (size_t)L.ldm);
-    L.off_eb = ws.plan(sizeof(double) * (size_t)L.ldn);
+    L.off_as = ws.plan((size_t)kPlanes * L.ldm * L.Kb);
+    L.off_bs = ws.plan((size_t)kPlanes * L.ldn * L.Kb);
+    L.off_ea = ws.plan(sizeof(double) * kProds * (size_t)L.ldm);
+    L.off_eb = ws.plan(sizeof(double) * kProds * (size_t)L.ldn);
     L.off_cocc = ws.plan(sizeof(double2) * (size_t)std::max(p->nvband, 1) * L.ldm);
     L.off_bmat = ws.plan(sizeof(double2) * (size_t)L.ldm * L.ldn);
     L.off_part = ws.plan(sizeof(double2) * 2 * (size_t)p->nfreq * p->ngpown * pole_chunks(p->ncouls));
@@ -917,7 +930,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
                   const float2 *aqsm, const float2 *aqsn, const float2 *ieps, const float2 *wt, const double *vcoul,
                   double2 *d_ach, double2 *d_asx, double2 *d_achs, hipStream_t s)
 {
-    const int nks = L.Kp / kKStep, nkh = L.Kp / kKHalf;
+    const int nkb = L.Kb / kKStep;
     PrepArgs pr;
     pr.aqsm = aqsm; pr.aqsn = aqsn; pr.vcoul = vcoul; pr.igp = d_igp;
     pr.as = static_cast<int8_t *>(ws.at(L.off_as));
@@ -926,14 +939,14 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     pr.rb = static_cast<double *>(ws.at(L.off_eb));
     pr.c_occ = static_cast<double2 *>(ws.at(L.off_cocc));
     pr.nb = p->number_bands; pr.nv = p->nvband; pr.C = p->ncouls; pr.G = p->ngpown;
-    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkh = nkh; pr.blocks_a = L.ldm / kPrepRowsA;
+    pr.ldm = L.ldm; pr.ldn = L.ldn; pr.nkb = nkb; pr.blocks_a = L.ldm / kPrepRowsA;
     hipLaunchKernelGGL(oz_prep_kernel, dim3(L.ldm / kPrepRowsA + L.ldn / kPrepColsB), dim3(256), 0, s, pr);
     FK_HIP(hipGetLastError());
 
     GemmArgs ga;
-    ga.as = pr.as; ga.bs = pr.bs;
+    ga.as = pr.as; ga.bs = pr.bs; ga.ra = pr.ra; ga.rb = pr.rb;
     ga.bmat = static_cast<double2 *>(ws.at(L.off_bmat));
-    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkh = nkh;
+    ga.ldm = L.ldm; ga.ldn = L.ldn; ga.nkb = nkb;
     ga.tiles_m = L.tiles_m; ga.tiles_n = L.tiles_n; ga.group_m = kGroupM;
     const int tiles = L.tiles_m * L.tiles_n;
     KernelTimer timer(o->kernel_ms, s);
@@ -943,7 +956,7 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     FK_TRY(timer.stop());
 
     PoleArgs pa;
-    pa.bmat = ga.bmat; pa.ra = pr.ra; pa.rb = pr.rb; pa.c_occ = pr.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
+    pa.bmat = ga.bmat; pa.c_occ = pr.c_occ; pa.aqsn = aqsn; pa.I_eps = ieps; pa.wtilde = wt; pa.igp = d_igp;
     pa.partials = static_cast<double2 *>(ws.at(L.off_part));
     pa.achs_part = d_achs ? static_cast<double2 *>(ws.at(L.off_achs)) : nullptr;
     pa.nv = p->nvband; pa.C = p->ncouls; pa.G = p->ngpown; pa.ldm = L.ldm; pa.ldn = L.ldn;

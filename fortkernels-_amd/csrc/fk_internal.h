@@ -112,7 +112,7 @@ int gpp_sym_launch(const fk_gpp_problem *p, const double2 *c, int ldc, int Kpad,
 // Mixed-precision GPP on int8 MFMA (complex64 inputs): workspace layout,
 // size limits, and the launch of its three kernels plus the reduction.
 struct OzLayout {
-    int ldm, ldn, Kp, tiles_m, tiles_n;
+    int ldm, ldn, Kb, tiles_m, tiles_n;   // Kb: bands padded to the GEMM step
     size_t off_as, off_bs, off_ea, off_eb, off_cocc, off_bmat, off_part, off_achs;
 };
 void gpp_oz_plan(const fk_gpp_problem *p, Scratch &ws, OzLayout &L);

@@ -67,7 +67,7 @@ def test_noalloc_equals_gpp_kernel_two_calls(fk, fko, precision, config):
     import torch
     assert torch.cuda.is_available(), "no ROCm device visible"
     cdt = torch.complex64 if precision == 1 else torch.complex128
-    big = fko.gpp_random(128, 2, 4096, 205, seed=21)   # fits the array in every form (int8 split: 20 MiB)
+    big = fko.gpp_random(128, 2, 4096, 205, seed=21)   # fits the array in every form (int8 split: 22.6 MiB)
     small = fko.gpp_random(40, 3, 1024, 51, nw=5, seed=22, wtilde_re=(3.0, 5.0))
     if precision == 1:
         for p in (big, small):
