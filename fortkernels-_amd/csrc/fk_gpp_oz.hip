@@ -833,7 +833,10 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecor
     const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
     const int local = b - first[p];
     if (local >= rec[p].gemm_tiles) return;
-    oz_gemm_tile(rec[p].ga, local);
+    // a copy in registers: the tile's direct loads are asm statements that
+    // clobber memory, after which fields read through rec would be reloaded
+    const GemmArgs ga = rec[p].ga;
+    oz_gemm_tile(ga, local);
 }
 
 __global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
