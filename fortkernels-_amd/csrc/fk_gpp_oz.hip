@@ -825,7 +825,8 @@ __device__ __forceinline__ int batch_problem(const int *first, int n, int b)
 __global__ __launch_bounds__(256) void oz_prep_batch_kernel(const OzRecord *rec, const int *first, int n)
 {
     const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
-    oz_prep_block(rec[p].pr, b - first[p]);
+    const PrepArgs pr = rec[p].pr;   // in registers: not reloaded after each barrier
+    oz_prep_block(pr, b - first[p]);
 }
 
 __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecord *rec, const int *first, int n)
@@ -842,6 +843,8 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecor
 __global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
 {
     const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
+    // by reference: a register copy of the arguments (with the pole
+    // parameters of up to 16 frequencies) made this kernel 1.7x slower
     oz_pole_block<false, kPoleStream | kPoleBlocks>(rec[p].pa, b - first[p]);
 }
 
