@@ -130,7 +130,7 @@ __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
     const int nblk = (a.nb + 15) / 16;
     double mx[kProds] = {0.0, 0.0, 0.0};
     for (int j = grp; j < nblk; j += NG)
-#pragma unroll
+#pragma unroll 8
         for (int b = 0; b < 16; ++b) {
             const double2 v = value(16 * j + b);
             mx[0] = fmax(mx[0], fabs(v.x));
@@ -152,30 +152,37 @@ __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
 
     int8_t *base = IS_A ? a.as : a.bs;
     const size_t blk = idx >> 5, nblks = (size_t)ld / 32;
-    for (int j = grp; j < a.nkb * 2; j += NG) {   // 16-band blocks of the padded K
-        int8_t q[kProds][kSlices][16];
-#pragma unroll
-        for (int b = 0; b < 16; ++b) {
-            const int n1 = 16 * j + b;
-            const double2 v = value(n1);
-            if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = v;
-            const double x[kProds] = {v.x, v.y, v.x + v.y};
-#pragma unroll
-            for (int pr = 0; pr < kProds; ++pr) {
-                int8_t sl[kSlices];
-                split4(x[pr], scale[pr], sl);
-#pragma unroll
-                for (int k = 0; k < kSlices; ++k) q[pr][k][b] = sl[k];
-            }
-        }
+    // 16-band blocks of the padded K, each in two halves of 8 bands (8 of
+    // the lane's 16 bytes per plane), which keeps the kernel at 4 waves per
+    // SIMD
+    for (int j = grp; j < a.nkb * 2; j += NG) {
         const int lane = (idx & 31) + 32 * (j & 1), kb = j >> 1;
 #pragma unroll
-        for (int pr = 0; pr < kProds; ++pr)
+        for (int h = 0; h < 2; ++h) {
+            int8_t q[kProds][kSlices][8];
 #pragma unroll
-            for (int k = 0; k < kSlices; ++k) {
-                int8_t *dst = base + (((size_t)(pr * kSlices + k) * nblks + blk) * a.nkb + kb) * kFrag + 16 * lane;
-                *reinterpret_cast<i4 *>(dst) = *reinterpret_cast<const i4 *>(q[pr][k]);
+            for (int b = 0; b < 8; ++b) {
+                const int n1 = 16 * j + 8 * h + b;
+                const double2 v = value(n1);
+                if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = v;
+                const double x[kProds] = {v.x, v.y, v.x + v.y};
+#pragma unroll
+                for (int pr = 0; pr < kProds; ++pr) {
+                    int8_t sl[kSlices];
+                    split4(x[pr], scale[pr], sl);
+#pragma unroll
+                    for (int k = 0; k < kSlices; ++k) q[pr][k][b] = sl[k];
+                }
             }
+#pragma unroll
+            for (int pr = 0; pr < kProds; ++pr)
+#pragma unroll
+                for (int k = 0; k < kSlices; ++k) {
+                    int8_t *dst = base + (((size_t)(pr * kSlices + k) * nblks + blk) * a.nkb + kb) * kFrag +
+                                  16 * lane + 8 * h;
+                    *reinterpret_cast<uint2 *>(dst) = *reinterpret_cast<const uint2 *>(q[pr][k]);
+                }
+        }
     }
 }
 
