@@ -286,8 +286,22 @@ __device__ __forceinline__ void wait_loads_barrier(int n)
 // has no register outputs.
 __device__ __forceinline__ void lds_load16(const int8_t *g, const int8_t *lds)
 {
-    const unsigned la = (unsigned)(uintptr_t)lds;   // the LDS offset: low half of the shared-aperture address
+    // the LDS offset (low half of the shared-aperture address), uniform
+    const unsigned la = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)lds);
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(la) : "memory", "m0");
+}
+
+// The same from a wave-uniform base (kept in SGPRs) plus a per-lane byte
+// offset: one VGPR serves every load of the kernel.
+__device__ __forceinline__ void lds_load16(const int8_t *base, unsigned voff, const int8_t *lds)
+{
+    const unsigned la = (unsigned)__builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)lds);
+    const uint64_t bu = (uint64_t)(uintptr_t)base;
+    // readfirstlane yields an int: widen through unsigned (no sign extension)
+    const uint64_t ub = ((uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)(bu >> 32)) << 32) |
+                        (uint64_t)(unsigned)__builtin_amdgcn_readfirstlane((unsigned)bu);
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(ub), "s"(la)
+                 : "memory", "m0");
 }
 
 struct StepFrags {
@@ -295,16 +309,25 @@ struct StepFrags {
     i4 b[kProds][kSlices];   // this wave's column block
 };
 
-// one 64x64 tile (workgroup b of the problem's launch)
-__device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
+// Tiles first, first + stride, ... (count >= 1 of them) of the launch order
+// (oz_tile), one after another on one workgroup.  The loads of a tile's
+// first two steps are issued during the previous tile's last two steps, so
+// only the workgroup's first tile waits for its operands from scratch.
+//
+// The tile's row and column scales (M / 127 of its 64 rows and 64 columns
+// for each matrix, 3 KB) travel with the operands: waves 0..2 each move one
+// 1 KB piece with a direct global->LDS load issued with the tile's step-0
+// loads, into one of two scale buffers (tile parity), so the waits that
+// cover the operands cover them too and the epilogue reads them from LDS.
+__device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int stride, int count)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
+    __shared__ __attribute__((aligned(16))) double scl[2][2 * kProds][kBM];   // [tile parity][ra 0..2, rb 0..2][64]
+    static_assert(kBM == kBN && kBM == 64, "the scale pieces assume 64-row and 64-column tiles");
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
-    int tm, tn;
-    oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
     const int nks = a.nkb;
 
     // this wave's 12 fragment loads per step: stage fragment f = 12 wave + k
@@ -313,19 +336,37 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
     // fragment apart.  Waves 0 and 1 move A, waves 2 and 3 B.
     const bool side_a = wave < 2;
     const size_t nblk = (size_t)(side_a ? a.ldm : a.ldn) / 32;
-    const int t2 = 2 * (side_a ? tm : tn);
-    const int8_t *vbase = (side_a ? a.as : a.bs) + 16 * lane;
-    auto src_off = [&](int k) -> size_t {   // uniform
+    const int8_t *uside = side_a ? a.as : a.bs;   // uniform
+    const unsigned voff = 16 * lane;
+    auto koff = [&](int k) -> size_t {   // uniform: fragment k of a step, from the tile's base
         const int f = (wave & 1) * kLoadsPerWave + k;
-        return (((size_t)(f >> 1) * nblk + t2 + (f & 1)) * a.nkb) * kFrag;
+        return ((size_t)(f >> 1) * nblk + (f & 1)) * a.nkb * kFrag;
     };
-    auto issue_one = [&](int s, int k) {
-        int8_t *dst = &lds[s % kStages][wave * kLoadsPerWave + k][0];
-        lds_load16(vbase + src_off(k) + (size_t)s * kFrag, dst);
+    // this wave's step-0 fragments of tile b
+    auto tile_src = [&](int b) -> const int8_t * {
+        int tm, tn;
+        oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
+        return uside + (size_t)(2 * (side_a ? tm : tn)) * a.nkb * kFrag;
     };
-    auto issue = [&](int s) {
+    // scale piece of tile b moved by wave w < 3: lanes 0..31 carry piece
+    // 2w, lanes 32..63 piece 2w+1 (pieces 0..2: ra rows, 3..5: rb columns),
+    // 16 bytes = 2 doubles per lane
+    auto issue_scales = [&](int b, int t) {
+        if (wave < kProds) {
+            int tm, tn;
+            oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
+            const int q = 2 * wave + (lane >> 5);
+            const double *g = q < kProds ? a.ra + (size_t)q * a.ldm + tm * kBM
+                                         : a.rb + (size_t)(q - kProds) * a.ldn + tn * kBN;
+            lds_load16(reinterpret_cast<const int8_t *>(g + 2 * (lane & 31)),
+                       reinterpret_cast<const int8_t *>(&scl[t & 1][2 * wave][0]));
+        }
+    };
+    auto stage = [&](int g) -> int8_t * { return &lds[g % kStages][0][0]; };
+    auto issue = [&](const int8_t *src, int g) {
 #pragma unroll
-        for (int k = 0; k < kLoadsPerWave; ++k) issue_one(s, k);
+        for (int k = 0; k < kLoadsPerWave; ++k)
+            lds_load16(src + koff(k), voff, stage(g) + (wave * kLoadsPerWave + k) * kFrag);
     };
     // LDS read #r (0..23) of a step's fragments: A (matrix r/8, slice
     // (r/2)%4) for even r, B for odd r
@@ -336,27 +377,20 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
         else
             f.b[pr][sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + pr * 8 + sl * 2 + wn) * kFrag + 16 * lane);
     };
-    auto stage = [&](int s) -> int8_t * { return &lds[s % kStages][0][0]; };
 
     i16v acc[kProds][kLevels];
-#pragma unroll
-    for (int pr = 0; pr < kProds; ++pr)
-#pragma unroll
-        for (int L = 0; L < kLevels; ++L) acc[pr][L] = i16v{};
 
-    // one step: MFMAs from `cur`; with LOADS, the loads of step s+2 over
-    // the first half; with NEXT, the mid-step wait for step s+1 and its
-    // fragments read into `nxt` over the second half
-    // The source (gsrc), the stage being loaded (wr) and the stage being
-    // read (rd) are restrict-qualified: after inlining, their accesses carry
-    // alias scopes, from which the compiler sees that the LDS reads of rd do
-    // not depend on the direct loads into wr issued in the same step, so it
-    // does not make the reads wait for those loads; the explicit wait above
-    // the reads covers the loads of step s+1.
-    auto step = [&](auto next_t, auto loads_t, const StepFrags &cur, StepFrags &nxt, int s,
-                    const int8_t *__restrict__ gsrc, int8_t *__restrict__ wr, const int8_t *__restrict__ rd) {
+    // the g-th step of this workgroup: MFMAs from `cur`; with LOADS, the
+    // loads from `src` into the stage of step g+2 over the first half (with
+    // SCALES, also tile b's scales); with NEXT, the mid-step wait for step
+    // g+1 and its fragments read into `nxt` over the second half
+    auto step = [&](auto next_t, auto loads_t, auto scales_t, const StepFrags &cur, StepFrags &nxt, int g,
+                    const int8_t *src, int b, int t) {
         constexpr bool NEXT = decltype(next_t)::value;
         constexpr bool LOADS = decltype(loads_t)::value;
+        constexpr bool SCALES = decltype(scales_t)::value;
+        int8_t *wr = stage(g + 2);
+        const int8_t *rd = stage(g + 1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int o = 0; o < kMfmasStep; ++o) {
@@ -365,8 +399,10 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
             acc[pr][ps.si + ps.sj] = mfma_i8(cur.a[pr][ps.si], cur.b[pr][ps.sj], acc[pr][ps.si + ps.sj]);
             __builtin_amdgcn_sched_barrier(0);
             if constexpr (LOADS) {
-                if (o < kLoadsPerWave)
-                    lds_load16(gsrc + src_off(o), wr + (wave * kLoadsPerWave + o) * kFrag);
+                if (o < kLoadsPerWave) lds_load16(src + koff(o), voff, wr + (wave * kLoadsPerWave + o) * kFrag);
+                if constexpr (SCALES) {
+                    if (o == kLoadsPerWave) issue_scales(b, t);
+                }
             }
             if constexpr (NEXT) {
                 if (o == kHalf - 1) wait_loads_barrier(LOADS ? 12 : 0);
@@ -381,61 +417,107 @@ __device__ __forceinline__ void oz_gemm_tile(const GemmArgs &a, int b)
     using yes = std::integral_constant<bool, true>;
     using no = std::integral_constant<bool, false>;
 
-    // the step count is even and at least 2 (the plan pads the bands to a
-    // multiple of 64), so the steps pair up with the two register sets
+    // The step count is even and at least 2 (the plan pads the bands to a
+    // multiple of 64), so the steps pair up with the two register sets and
+    // every tile starts with its step-0 fragments in h0.
+    //
+    // Waits: vmcnt counts this wave's direct loads and its B stores; the
+    // loads complete in issue order.  A mid-step wait for step g+1 allows 12
+    // (the loads of step g+2, issued after it) or, with no loads issued in
+    // the step, 0 outstanding; the scale load of a step (waves 0..2) is
+    // issued after that step's 12 loads, so it only makes the count of later
+    // operations larger.  The B stores of an epilogue are issued after the
+    // next tile's loads of steps 0 and 1, so they are done by the next
+    // tile's first mid-step wait.
     StepFrags h0, h1;
-    issue(0);
-    issue(1);
-    // the tile's row and column scales, loaded now so that the epilogue
-    // does not wait on them; the row scales go through LDS
-    const int col = tn * kBN + wn * 32 + (lane & 31);
-    const double rav = tid < kProds * kBM ? a.ra[(size_t)(tid / kBM) * a.ldm + tm * kBM + tid % kBM] : 0.0;
-    double rbc[kProds];
-#pragma unroll
-    for (int pr = 0; pr < kProds; ++pr) rbc[pr] = a.rb[(size_t)pr * a.ldn + col];
+    const int8_t *src = tile_src(first);
+    issue(src, 0);
+    issue_scales(first, 0);
+    issue(src + kFrag, 1);
     wait_loads_barrier(12);
 #pragma unroll
     for (int r = 0; r < 2 * kPlanes; ++r) read_frag(stage(0), h0, r);
-    int s = 0;
-    // steady state: every step has a next step and loads to issue
-    auto src = [&](int s) { return vbase + (size_t)s * kFrag; };
-    for (; s + 2 < nks; s += 2) {
-        step(yes{}, yes{}, h0, h1, s, src(s + 2), stage(s + 2), stage(s + 1));
-        step(yes{}, yes{}, h1, h0, s + 1, src(s + 3), stage(s + 3), stage(s + 2));
-    }
-    // the last two steps
-    step(yes{}, no{}, h0, h1, s, src(s), stage(s + 2), stage(s + 1));
-    step(no{}, no{}, h1, h0, s + 1, src(s), stage(s + 3), stage(s + 2));
 
-    // B in FP64: P = (sum_L 254^-L acc_L) (M_row / 127) (M_col / 127) for
-    // each product (int32 -> FP64 is exact), then re = P0 - P1 and
-    // im = P2 - P0 - P1
-    // B is written once and read once by the pole kernel: streaming
-    // (nontemporal) stores keep it from displacing the operand slices in L2
-    __shared__ double sra[kProds][kBM];
-    if (tid < kProds * kBM) sra[tid / kBM][tid % kBM] = rav;
-    __syncthreads();
     typedef double d2v __attribute__((ext_vector_type(2)));
     const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0};
+    int g = 0, b = first;
+    for (int t = 0; t < count; ++t, b += stride, g += nks) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        double P[kProds];
+        for (int pr = 0; pr < kProds; ++pr)
 #pragma unroll
-        for (int pr = 0; pr < kProds; ++pr) {
-            double v = 0.0;
-#pragma unroll
-            for (int L = kLevels - 1; L >= 0; --L) v += W[L] * (double)acc[pr][L][j];
-            P[pr] = v * sra[pr][r] * rbc[pr];
+            for (int L = 0; L < kLevels; ++L) acc[pr][L] = i16v{};
+        int s = 0;
+        for (; s + 2 < nks; s += 2) {
+            step(yes{}, yes{}, no{}, h0, h1, g + s, src + (size_t)(s + 2) * kFrag, b, t);
+            step(yes{}, yes{}, no{}, h1, h0, g + s + 1, src + (size_t)(s + 3) * kFrag, b, t);
         }
-        const d2v bv = {P[0] - P[1], P[2] - P[0] - P[1]};
-        __builtin_nontemporal_store(bv, reinterpret_cast<d2v *>(&a.bmat[(size_t)(tm * kBM + r) * a.ldn + col]));
+        // the last two steps load the next tile's first two (the workgroup's
+        // last tile loads its own again: one code path, data unused)
+        const bool more = t + 1 < count;
+        const int bn = more ? b + stride : b;
+        const int8_t *nsrc = tile_src(bn);
+        step(yes{}, yes{}, yes{}, h0, h1, g + s, nsrc, bn, t + 1);
+        step(no{}, yes{}, no{}, h1, h0, g + s + 1, nsrc + kFrag, b, t);
+        src = nsrc;
+
+        // B in FP64: P = (sum_L 254^-L acc_L) (M_row / 127) (M_col / 127)
+        // for each product (int32 -> FP64 is exact), then re = P0 - P1 and
+        // im = P2 - P0 - P1.  All of B is formed first (the accumulators
+        // are then free), then the next tile's step-0 fragments are waited
+        // for and read, then B is stored: the wait counts loads only.  The
+        // epilogue ends with a barrier, after which this tile's scale buffer
+        // may be refilled for the tile after next.  B is written once and
+        // read once by the pole kernel: streaming (nontemporal) stores keep
+        // it from displacing the operand slices in L2.
+        int tm, tn;
+        oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
+        const double *sc = &scl[t & 1][0][0];
+        double rbc[kProds];
+#pragma unroll
+        for (int pr = 0; pr < kProds; ++pr) rbc[pr] = sc[(kProds + pr) * kBM + wn * 32 + (lane & 31)];
+        const int col = tn * kBN + wn * 32 + (lane & 31);
+        d2v bv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            double P[kProds];
+#pragma unroll
+            for (int pr = 0; pr < kProds; ++pr) {
+                double v = 0.0;
+#pragma unroll
+                for (int L = kLevels - 1; L >= 0; --L) v += W[L] * (double)acc[pr][L][j];
+                P[pr] = v * sc[pr * kBM + r] * rbc[pr];
+            }
+            bv[j] = d2v{P[0] - P[1], P[2] - P[0] - P[1]};
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            // later loads than the next tile's step 0: its step 1 (12) and,
+            // for waves 0..2, the scale piece
+            wait_loads_barrier(12);
+#pragma unroll
+            for (int r = 0; r < 2 * kPlanes; ++r) read_frag(stage(g + nks), h0, r);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            __builtin_nontemporal_store(bv[j], reinterpret_cast<d2v *>(&a.bmat[(size_t)(tm * kBM + r) * a.ldn + col]));
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
+    // the last tile's loads for a next tile may still be in flight: they
+    // write LDS, which must not be released (to another workgroup) before
+    // they land
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a)
+// one workgroup per CU, each walking its share of the tiles
+__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a, int tiles)
 {
-    oz_gemm_tile(a, blockIdx.x);
+    const int first = blockIdx.x, stride = gridDim.x;
+    if (first >= tiles) return;
+    oz_gemm_tiles(a, first, stride, (tiles - 1 - first) / stride + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -844,7 +926,7 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecor
     // a copy in registers: the tile's direct loads are asm statements that
     // clobber memory, after which fields read through rec would be reloaded
     const GemmArgs ga = rec[p].ga;
-    oz_gemm_tile(ga, local);
+    oz_gemm_tiles(ga, local, 1, 1);
 }
 
 __global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
@@ -964,7 +1046,10 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     const int tiles = L.tiles_m * L.tiles_n;
     KernelTimer timer(o->kernel_ms, s);
     FK_TRY(timer.start());
-    hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(tiles), dim3(256), 0, s, ga);
+    int dev = 0, ncu = 0;
+    FK_HIP(hipGetDevice(&dev));
+    FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(std::min(tiles, std::max(ncu, 1))), dim3(256), 0, s, ga, tiles);
     FK_HIP(hipGetLastError());
     FK_TRY(timer.stop());
 
