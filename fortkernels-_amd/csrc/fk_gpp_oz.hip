@@ -152,17 +152,18 @@ __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
 
     int8_t *base = IS_A ? a.as : a.bs;
     const size_t blk = idx >> 5, nblks = (size_t)ld / 32;
-    // 16-band blocks of the padded K, each in two halves of 8 bands (8 of
-    // the lane's 16 bytes per plane), which keeps the kernel at 4 waves per
-    // SIMD
+    // 16-band blocks of the padded K, split in two halves of 8 bands (a
+    // scheduling barrier keeps each half's loads and splits together, so
+    // only 8 values are live at once; 4 waves per SIMD), then one 16-byte
+    // store per plane
     for (int j = grp; j < a.nkb * 2; j += NG) {
         const int lane = (idx & 31) + 32 * (j & 1), kb = j >> 1;
+        uint32_t q[kProds][kSlices][4] = {};   // the 16 bytes of each plane, packed
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            int8_t q[kProds][kSlices][8];
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
-                const int n1 = 16 * j + 8 * h + b;
+                const int n1 = 16 * j + 8 * h + b, byte = 8 * h + b;
                 const double2 v = value(n1);
                 if (IS_A && n1 < a.nv) a.c_occ[(size_t)n1 * a.ldm + idx] = v;
                 const double x[kProds] = {v.x, v.y, v.x + v.y};
@@ -171,18 +172,19 @@ __device__ __forceinline__ void prep_slab(const PrepArgs &a, int slab)
                     int8_t sl[kSlices];
                     split4(x[pr], scale[pr], sl);
 #pragma unroll
-                    for (int k = 0; k < kSlices; ++k) q[pr][k][b] = sl[k];
+                    for (int k = 0; k < kSlices; ++k)
+                        q[pr][k][byte >> 2] |= (uint32_t)(uint8_t)sl[k] << (8 * (byte & 3));
                 }
             }
-#pragma unroll
-            for (int pr = 0; pr < kProds; ++pr)
-#pragma unroll
-                for (int k = 0; k < kSlices; ++k) {
-                    int8_t *dst = base + (((size_t)(pr * kSlices + k) * nblks + blk) * a.nkb + kb) * kFrag +
-                                  16 * lane + 8 * h;
-                    *reinterpret_cast<uint2 *>(dst) = *reinterpret_cast<const uint2 *>(q[pr][k]);
-                }
+            __builtin_amdgcn_sched_barrier(0);
         }
+#pragma unroll
+        for (int pr = 0; pr < kProds; ++pr)
+#pragma unroll
+            for (int k = 0; k < kSlices; ++k) {
+                int8_t *dst = base + (((size_t)(pr * kSlices + k) * nblks + blk) * a.nkb + kb) * kFrag + 16 * lane;
+                *reinterpret_cast<uint4 *>(dst) = make_uint4(q[pr][k][0], q[pr][k][1], q[pr][k][2], q[pr][k][3]);
+            }
     }
 }
 
@@ -319,6 +321,7 @@ struct StepFrags {
 // 1 KB piece with a direct global->LDS load issued with the tile's step-0
 // loads, into one of two scale buffers (tile parity), so the waits that
 // cover the operands cover them too and the epilogue reads them from LDS.
+template <bool PERSIST>
 __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int stride, int count)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
@@ -451,13 +454,21 @@ __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int 
             step(yes{}, yes{}, no{}, h0, h1, g + s, src + (size_t)(s + 2) * kFrag, b, t);
             step(yes{}, yes{}, no{}, h1, h0, g + s + 1, src + (size_t)(s + 3) * kFrag, b, t);
         }
-        // the last two steps load the next tile's first two (the workgroup's
-        // last tile loads its own again: one code path, data unused)
-        const bool more = t + 1 < count;
-        const int bn = more ? b + stride : b;
-        const int8_t *nsrc = tile_src(bn);
-        step(yes{}, yes{}, yes{}, h0, h1, g + s, nsrc, bn, t + 1);
-        step(no{}, yes{}, no{}, h1, h0, g + s + 1, nsrc + kFrag, b, t);
+        const int8_t *nsrc = src;
+        if constexpr (PERSIST) {
+            // the last two steps load the next tile's first two (the
+            // workgroup's last tile loads its own again: one code path, data
+            // unused; two paths made the compiler spill)
+            const bool more = t + 1 < count;
+            const int bn = more ? b + stride : b;
+            nsrc = tile_src(bn);
+            step(yes{}, yes{}, yes{}, h0, h1, g + s, nsrc, bn, t + 1);
+            step(no{}, yes{}, no{}, h1, h0, g + s + 1, nsrc + kFrag, b, t);
+        } else {
+            // one tile: no loads in the last two steps
+            step(yes{}, no{}, no{}, h0, h1, g + s, src, b, t);
+            step(no{}, no{}, no{}, h1, h0, g + s + 1, src, b, t);
+        }
         src = nsrc;
 
         // B in FP64: P = (sum_L 254^-L acc_L) (M_row / 127) (M_col / 127)
@@ -491,7 +502,7 @@ __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int 
             bv[j] = d2v{P[0] - P[1], P[2] - P[0] - P[1]};
         }
         __builtin_amdgcn_sched_barrier(0);
-        {
+        if constexpr (PERSIST) {
             // later loads than the next tile's step 0: its step 1 (12) and,
             // for waves 0..2, the scale piece
             wait_loads_barrier(12);
@@ -517,7 +528,7 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a, int til
 {
     const int first = blockIdx.x, stride = gridDim.x;
     if (first >= tiles) return;
-    oz_gemm_tiles(a, first, stride, (tiles - 1 - first) / stride + 1);
+    oz_gemm_tiles<true>(a, first, stride, (tiles - 1 - first) / stride + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -926,7 +937,7 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecor
     // a copy in registers: the tile's direct loads are asm statements that
     // clobber memory, after which fields read through rec would be reloaded
     const GemmArgs ga = rec[p].ga;
-    oz_gemm_tiles(ga, local, 1, 1);
+    oz_gemm_tiles<false>(ga, local, 1, 1);
 }
 
 __global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
