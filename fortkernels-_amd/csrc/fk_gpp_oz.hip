@@ -202,8 +202,8 @@ __global__ __launch_bounds__(256) void oz_prep_kernel(PrepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// int8 MFMA GEMM.  64x64 workgroup tile, 2x2 waves of one 32x32 block
-// each.  32x32x32 int8 MFMA: lane l supplies 16 bytes of row l%32 of A and
+// int8 MFMA GEMM.  64x64 workgroup tile of 2x2 blocks of 32x32.
+// 32x32x32 int8 MFMA: lane l supplies 16 bytes of row l%32 of A and
 // of column l%32 of B at the same 16 bands; accumulator register j of lane
 // l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32.
 //
@@ -218,28 +218,27 @@ __global__ __launch_bounds__(256) void oz_prep_kernel(PrepArgs a)
 //
 // LDS ring of kStages steps of 32 bands; a stage holds the workgroup's 24 A
 // fragments ([matrix][slice][row block]) and 24 B fragments
-// ([matrix][slice][column block]), 48 KB.  Each wave moves 12 fragments per
-// step with direct global->LDS loads, issued over the first half of step s
-// for step s+2, into the stage step s-1 used.  At the middle of step s the
-// wave waits for its loads of step s+1 and passes a barrier that covers
-// every wave's loads; the second half reads step s+1's fragments into the
-// other register set.  Scheduling barriers hold the order; two MFMAs into
-// one accumulator are at least nine apart.
+// ([matrix][slice][column block]), 48 KB.  Twelve waves, three per SIMD:
+// wave w owns the 32x32 block w % 4 and the product w / 4.  Each wave moves
+// 4 fragments per step with direct global->LDS loads, issued over the first
+// MFMAs of step s for step s+2, into the stage step s-1 used; after its
+// fifth MFMA it waits for its loads of step s+1 and passes a barrier that
+// covers every wave's loads, and its next MFMAs carry the 8 LDS reads of
+// its fragments of step s+1 into the other register set.  With three waves
+// per SIMD the MFMA pipe is fed by whichever wave is ready (one wave per
+// SIMD, each computing all three products of its block, left the pipes
+// busy 47% of the time).
 // ---------------------------------------------------------------------------
 constexpr int kBM = 64, kBN = 64, kGroupM = 4, kStages = 3;
 constexpr int kFragsA = kPlanes * (kBM / 32), kFragsB = kPlanes * (kBN / 32);   // 24, 24
 constexpr int kFragsPerStage = kFragsA + kFragsB;          // 48
-constexpr int kLoadsPerWave = kFragsPerStage / 4;          // 12
 
-// slice pairs of a step, each followed by its three products; levels
-// si + sj: 3 2 1 3 2 1 3 2 0 3
+// slice pairs of a wave's step in MFMA order; levels si + sj: 3 2 1 3 2 1
+// 3 2 0 3 (an accumulator is reused at most every third MFMA)
 struct PairStep { int si, sj; };
 constexpr int kPairs = 10;
 constexpr PairStep kOrder[kPairs] = {{0, 3}, {0, 2}, {0, 1}, {1, 2}, {1, 1},
                                      {1, 0}, {2, 1}, {2, 0}, {0, 0}, {3, 0}};
-constexpr int kMfmasStep = kPairs * kProds;   // 30
-constexpr int kHalf = 15;                     // MFMAs before the mid-step wait
-static_assert(kLoadsPerWave <= kHalf && 2 * (kMfmasStep - kHalf) >= 2 * kPlanes, "schedule slots");
 
 struct GemmArgs {
     const int8_t *as, *bs;
@@ -272,13 +271,6 @@ __device__ __forceinline__ void oz_tile(int b, int tiles_m, int tiles_n, int gro
     tn = u / gm;
 }
 
-// wait until at most n (12 or 0) of this wave's fragment loads are
-// outstanding and its LDS reads are done, then pass the workgroup barrier
-__device__ __forceinline__ void wait_loads_barrier(int n)
-{
-    if (n == 12) asm volatile("s_waitcnt vmcnt(12) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 // One direct global->LDS load of 16 bytes per lane: lane l's bytes land at
 // LDS byte address lds + 16 l.  Written as inline asm so that the compiler's
@@ -306,54 +298,77 @@ __device__ __forceinline__ void lds_load16(const int8_t *base, unsigned voff, co
                  : "memory", "m0");
 }
 
-struct StepFrags {
-    i4 a[kProds][kSlices];   // this wave's row block
-    i4 b[kProds][kSlices];   // this wave's column block
+// ---------------------------------------------------------------------------
+// The same 64x64 tile with 12 waves, three per SIMD: wave w owns the 32x32
+// block w % 4 and the product w / 4 (4 accumulators, 10 MFMAs per step).
+// The LDS ring and its layout are those above; each wave moves 4 of a
+// step's 48 fragments and reads the 8 of its block and product.  With three
+// waves per SIMD the MFMA pipe is fed by whichever wave is ready.  The
+// epilogue meets the three products of a block in the (then free) ring:
+// the waves of products 1 and 2 leave P there, those of product 0 form B.
+// ---------------------------------------------------------------------------
+constexpr int kWaves12 = 12;
+constexpr int kLoads12 = kFragsPerStage / kWaves12;   // 4
+constexpr int kWait12 = 4;                            // MFMA after which a wave waits
+// a wave's 8 fragments in the order of their first use by its step
+struct Frag8 { bool a; int sl; };
+constexpr Frag8 kRead12[2 * kSlices] = {{true, 0}, {false, 3}, {false, 2}, {false, 1},
+                                        {true, 1}, {false, 0}, {true, 2}, {true, 3}};
+
+__device__ __forceinline__ void wait_loads_barrier4(int n)
+{
+    if (n == 4) asm volatile("s_waitcnt vmcnt(4) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+struct Frags12 {
+    i4 a[kSlices];   // this wave's product, row block
+    i4 b[kSlices];   // this wave's product, column block
 };
 
-// Tiles first, first + stride, ... (count >= 1 of them) of the launch order
-// (oz_tile), one after another on one workgroup.  The loads of a tile's
-// first two steps are issued during the previous tile's last two steps, so
-// only the workgroup's first tile waits for its operands from scratch.
-//
-// The tile's row and column scales (M / 127 of its 64 rows and 64 columns
-// for each matrix, 3 KB) travel with the operands: waves 0..2 each move one
-// 1 KB piece with a direct global->LDS load issued with the tile's step-0
-// loads, into one of two scale buffers (tile parity), so the waits that
-// cover the operands cover them too and the epilogue reads them from LDS.
+// Tiles first, first + stride, ... (count >= 1 of them) in launch order on
+// one workgroup.  With PERSIST, the last two steps of a tile load the first
+// two of the next (the last tile reloads its own: one code path), and the
+// tile's scales travel as three more direct loads (waves 0..2, 1 KB each)
+// into a double-buffered LDS area; without it (one tile per workgroup) the
+// last two steps load nothing.  The epilogue forms this wave's scaled P,
+// waits for and reads the next tile's step-0 fragments, then meets the
+// three products of its block in the ring stage no load is writing (two
+// rounds of 8 rows), and the product-0 waves store B.
 template <bool PERSIST>
-__device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int stride, int count)
+__device__ __forceinline__ void oz_gemm12_tiles(const GemmArgs &a, int first, int stride, int count)
 {
     __shared__ __attribute__((aligned(16))) int8_t lds[kStages][kFragsPerStage][kFrag];
     __shared__ __attribute__((aligned(16))) double scl[2][2 * kProds][kBM];   // [tile parity][ra 0..2, rb 0..2][64]
-    static_assert(kBM == kBN && kBM == 64, "the scale pieces assume 64-row and 64-column tiles");
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 1, wn = wave >> 1;
+    const int blk = wave & 3, pr = wave >> 2;
+    const int wm = blk & 1, wn = blk >> 1;
     const int nks = a.nkb;
 
-    // this wave's 12 fragment loads per step: stage fragment f = 12 wave + k
-    // is A (plane f/2, row block 2 tm + f%2) for f < 24, else B (plane
-    // (f-24)/2, column block 2 tn + f%2); consecutive steps are one
-    // fragment apart.  Waves 0 and 1 move A, waves 2 and 3 B.
-    const bool side_a = wave < 2;
+    // this wave's loads: stage fragment f = 4 wave + k, A (plane f/2, row
+    // block 2 tm + f%2) for f < 24, else B (plane (f-24)/2, column block
+    // 2 tn + f%2); waves 0..5 move A, 6..11 B.  koff: from the tile's base.
+    const bool side_a = kLoads12 * wave < kFragsA;
     const size_t nblk = (size_t)(side_a ? a.ldm : a.ldn) / 32;
-    const int8_t *uside = side_a ? a.as : a.bs;   // uniform
-    const unsigned voff = 16 * lane;
-    auto koff = [&](int k) -> size_t {   // uniform: fragment k of a step, from the tile's base
-        const int f = (wave & 1) * kLoadsPerWave + k;
-        return ((size_t)(f >> 1) * nblk + (f & 1)) * a.nkb * kFrag;
+    const int8_t *uside = side_a ? a.as : a.bs;
+    auto koff = [&](int k) -> size_t {
+        const int q = kLoads12 * wave + k - (side_a ? 0 : kFragsA);
+        return ((size_t)(q >> 1) * nblk + (q & 1)) * a.nkb * kFrag;
     };
-    // this wave's step-0 fragments of tile b
     auto tile_src = [&](int b) -> const int8_t * {
         int tm, tn;
         oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
         return uside + (size_t)(2 * (side_a ? tm : tn)) * a.nkb * kFrag;
     };
-    // scale piece of tile b moved by wave w < 3: lanes 0..31 carry piece
-    // 2w, lanes 32..63 piece 2w+1 (pieces 0..2: ra rows, 3..5: rb columns),
-    // 16 bytes = 2 doubles per lane
+    const unsigned voff = 16 * lane;
+    auto stage = [&](int g) -> int8_t * { return &lds[g % kStages][0][0]; };
+    auto issue = [&](const int8_t *src, int g, int k) {
+        lds_load16(src + koff(k), voff, stage(g) + (kLoads12 * wave + k) * kFrag);
+    };
+    // scale pieces of tile b (waves 0..2): lanes 0..31 piece 2w, lanes
+    // 32..63 piece 2w+1 (0..2 ra rows, 3..5 rb columns), 2 doubles a lane
     auto issue_scales = [&](int b, int t) {
         if (wave < kProds) {
             int tm, tn;
@@ -365,53 +380,40 @@ __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int 
                        reinterpret_cast<const int8_t *>(&scl[t & 1][2 * wave][0]));
         }
     };
-    auto stage = [&](int g) -> int8_t * { return &lds[g % kStages][0][0]; };
-    auto issue = [&](const int8_t *src, int g) {
-#pragma unroll
-        for (int k = 0; k < kLoadsPerWave; ++k)
-            lds_load16(src + koff(k), voff, stage(g) + (wave * kLoadsPerWave + k) * kFrag);
-    };
-    // LDS read #r (0..23) of a step's fragments: A (matrix r/8, slice
-    // (r/2)%4) for even r, B for odd r
-    auto read_frag = [&](const int8_t *st, StepFrags &f, int r) {
-        const int pr = r >> 3, sl = (r >> 1) & 3;
-        if ((r & 1) == 0)
-            f.a[pr][sl] = *reinterpret_cast<const i4 *>(st + (pr * 8 + sl * 2 + wm) * kFrag + 16 * lane);
-        else
-            f.b[pr][sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + pr * 8 + sl * 2 + wn) * kFrag + 16 * lane);
+    auto read_frag = [&](const int8_t *st, Frags12 &f, int r) {
+        const Frag8 fr = kRead12[r];
+        if (fr.a) f.a[fr.sl] = *reinterpret_cast<const i4 *>(st + (pr * 8 + fr.sl * 2 + wm) * kFrag + 16 * lane);
+        else f.b[fr.sl] = *reinterpret_cast<const i4 *>(st + (kFragsA + pr * 8 + fr.sl * 2 + wn) * kFrag + 16 * lane);
     };
 
-    i16v acc[kProds][kLevels];
+    i16v acc[kLevels];
 
-    // the g-th step of this workgroup: MFMAs from `cur`; with LOADS, the
-    // loads from `src` into the stage of step g+2 over the first half (with
-    // SCALES, also tile b's scales); with NEXT, the mid-step wait for step
-    // g+1 and its fragments read into `nxt` over the second half
-    auto step = [&](auto next_t, auto loads_t, auto scales_t, const StepFrags &cur, StepFrags &nxt, int g,
+    // the g-th step of this workgroup: MFMAs from `cur`; with LOADS, the 4
+    // loads from `src` into the stage of step g+2 (and with SCALES tile
+    // b's scales after them); with NEXT, the wait for step g+1 after MFMA
+    // kWait12 and its fragments read into `nxt`
+    auto step = [&](auto next_t, auto loads_t, auto scales_t, const Frags12 &cur, Frags12 &nxt, int g,
                     const int8_t *src, int b, int t) {
         constexpr bool NEXT = decltype(next_t)::value;
         constexpr bool LOADS = decltype(loads_t)::value;
         constexpr bool SCALES = decltype(scales_t)::value;
-        int8_t *wr = stage(g + 2);
-        const int8_t *rd = stage(g + 1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int o = 0; o < kMfmasStep; ++o) {
-            const PairStep ps = kOrder[o / kProds];
-            const int pr = o % kProds;
-            acc[pr][ps.si + ps.sj] = mfma_i8(cur.a[pr][ps.si], cur.b[pr][ps.sj], acc[pr][ps.si + ps.sj]);
+        for (int o = 0; o < kPairs; ++o) {
+            const PairStep ps = kOrder[o];
+            acc[ps.si + ps.sj] = mfma_i8(cur.a[ps.si], cur.b[ps.sj], acc[ps.si + ps.sj]);
             __builtin_amdgcn_sched_barrier(0);
             if constexpr (LOADS) {
-                if (o < kLoadsPerWave) lds_load16(src + koff(o), voff, wr + (wave * kLoadsPerWave + o) * kFrag);
+                if (o < kLoads12) issue(src, g + 2, o);
                 if constexpr (SCALES) {
-                    if (o == kLoadsPerWave) issue_scales(b, t);
+                    if (o == kLoads12 - 1) issue_scales(b, t);
                 }
             }
             if constexpr (NEXT) {
-                if (o == kHalf - 1) wait_loads_barrier(LOADS ? 12 : 0);
-                if (o >= kHalf && o < kHalf + kPlanes) {
-                    read_frag(rd, nxt, 2 * (o - kHalf));
-                    read_frag(rd, nxt, 2 * (o - kHalf) + 1);
+                if (o == kWait12) wait_loads_barrier4(LOADS ? 4 : 0);
+                if (o > kWait12 && o <= kWait12 + kSlices) {
+                    read_frag(stage(g + 1), nxt, 2 * (o - kWait12 - 1));
+                    read_frag(stage(g + 1), nxt, 2 * (o - kWait12 - 1) + 1);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -420,35 +422,28 @@ __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int 
     using yes = std::integral_constant<bool, true>;
     using no = std::integral_constant<bool, false>;
 
-    // The step count is even and at least 2 (the plan pads the bands to a
-    // multiple of 64), so the steps pair up with the two register sets and
-    // every tile starts with its step-0 fragments in h0.
-    //
-    // Waits: vmcnt counts this wave's direct loads and its B stores; the
-    // loads complete in issue order.  A mid-step wait for step g+1 allows 12
-    // (the loads of step g+2, issued after it) or, with no loads issued in
-    // the step, 0 outstanding; the scale load of a step (waves 0..2) is
-    // issued after that step's 12 loads, so it only makes the count of later
-    // operations larger.  The B stores of an epilogue are issued after the
-    // next tile's loads of steps 0 and 1, so they are done by the next
-    // tile's first mid-step wait.
-    StepFrags h0, h1;
+    // Waits: vmcnt counts this wave's direct loads (4 a step, plus one
+    // scale piece for waves 0..2 issued after a step's loads) and B stores;
+    // the loads complete in issue order, and a wait allows only the 4 loads
+    // issued after the step waited for, so it is correct whatever the
+    // stores do.
+    Frags12 h0, h1;
     const int8_t *src = tile_src(first);
-    issue(src, 0);
-    issue_scales(first, 0);
-    issue(src + kFrag, 1);
-    wait_loads_barrier(12);
 #pragma unroll
-    for (int r = 0; r < 2 * kPlanes; ++r) read_frag(stage(0), h0, r);
+    for (int k = 0; k < kLoads12; ++k) issue(src, 0, k);
+    issue_scales(first, 0);
+#pragma unroll
+    for (int k = 0; k < kLoads12; ++k) issue(src + kFrag, 1, k);
+    wait_loads_barrier4(4);
+#pragma unroll
+    for (int r = 0; r < 2 * kSlices; ++r) read_frag(stage(0), h0, r);
 
     typedef double d2v __attribute__((ext_vector_type(2)));
     const double W[kLevels] = {1.0, 1.0 / 254.0, 1.0 / 64516.0, 1.0 / 16387064.0};
     int g = 0, b = first;
     for (int t = 0; t < count; ++t, b += stride, g += nks) {
 #pragma unroll
-        for (int pr = 0; pr < kProds; ++pr)
-#pragma unroll
-            for (int L = 0; L < kLevels; ++L) acc[pr][L] = i16v{};
+        for (int L = 0; L < kLevels; ++L) acc[L] = i16v{};
         int s = 0;
         for (; s + 2 < nks; s += 2) {
             step(yes{}, yes{}, no{}, h0, h1, g + s, src + (size_t)(s + 2) * kFrag, b, t);
@@ -456,79 +451,81 @@ __device__ __forceinline__ void oz_gemm_tiles(const GemmArgs &a, int first, int 
         }
         const int8_t *nsrc = src;
         if constexpr (PERSIST) {
-            // the last two steps load the next tile's first two (the
-            // workgroup's last tile loads its own again: one code path, data
-            // unused; two paths made the compiler spill)
             const bool more = t + 1 < count;
             const int bn = more ? b + stride : b;
             nsrc = tile_src(bn);
             step(yes{}, yes{}, yes{}, h0, h1, g + s, nsrc, bn, t + 1);
             step(no{}, yes{}, no{}, h1, h0, g + s + 1, nsrc + kFrag, b, t);
         } else {
-            // one tile: no loads in the last two steps
             step(yes{}, no{}, no{}, h0, h1, g + s, src, b, t);
             step(no{}, no{}, no{}, h1, h0, g + s + 1, src, b, t);
         }
         src = nsrc;
 
-        // B in FP64: P = (sum_L 254^-L acc_L) (M_row / 127) (M_col / 127)
-        // for each product (int32 -> FP64 is exact), then re = P0 - P1 and
-        // im = P2 - P0 - P1.  All of B is formed first (the accumulators
-        // are then free), then the next tile's step-0 fragments are waited
-        // for and read, then B is stored: the wait counts loads only.  The
-        // epilogue ends with a barrier, after which this tile's scale buffer
-        // may be refilled for the tile after next.  B is written once and
-        // read once by the pole kernel: streaming (nontemporal) stores keep
-        // it from displacing the operand slices in L2.
+        // P of this wave's product, scaled
         int tm, tn;
         oz_tile(b, a.tiles_m, a.tiles_n, a.group_m, tm, tn);
         const double *sc = &scl[t & 1][0][0];
-        double rbc[kProds];
-#pragma unroll
-        for (int pr = 0; pr < kProds; ++pr) rbc[pr] = sc[(kProds + pr) * kBM + wn * 32 + (lane & 31)];
-        const int col = tn * kBN + wn * 32 + (lane & 31);
-        d2v bv[16];
+        const double rbc = sc[(kProds + pr) * kBM + wn * 32 + (lane & 31)];
+        double P[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-            double P[kProds];
+            double v = 0.0;
 #pragma unroll
-            for (int pr = 0; pr < kProds; ++pr) {
-                double v = 0.0;
-#pragma unroll
-                for (int L = kLevels - 1; L >= 0; --L) v += W[L] * (double)acc[pr][L][j];
-                P[pr] = v * sc[pr * kBM + r] * rbc[pr];
-            }
-            bv[j] = d2v{P[0] - P[1], P[2] - P[0] - P[1]};
+            for (int L = kLevels - 1; L >= 0; --L) v += W[L] * (double)acc[L][j];
+            P[j] = v * sc[pr * kBM + r] * rbc;
         }
         __builtin_amdgcn_sched_barrier(0);
+        // the next tile's step-0 fragments (PERSIST), or just a barrier:
+        // after it every wave is done with the ring of this tile
         if constexpr (PERSIST) {
-            // later loads than the next tile's step 0: its step 1 (12) and,
-            // for waves 0..2, the scale piece
-            wait_loads_barrier(12);
+            wait_loads_barrier4(4);
 #pragma unroll
-            for (int r = 0; r < 2 * kPlanes; ++r) read_frag(stage(g + nks), h0, r);
+            for (int r = 0; r < 2 * kSlices; ++r) read_frag(stage(g + nks), h0, r);
+        } else {
+            wait_loads_barrier4(0);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        // the stage of this tile's last step is free (the next tile's steps
+        // 0 and 1 occupy the other two): the products meet there, 8 rows a
+        // round, [2 products][4 blocks][8][64] doubles = 32 KB
+        double *X = reinterpret_cast<double *>(stage(g + nks - 1));
+        const int col = tn * kBN + wn * 32 + (lane & 31);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-            __builtin_nontemporal_store(bv[j], reinterpret_cast<d2v *>(&a.bmat[(size_t)(tm * kBM + r) * a.ldn + col]));
+        for (int h = 0; h < 2; ++h) {
+            if (pr > 0) {
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) X[(((pr - 1) * 4 + blk) * 8 + jj) * 64 + lane] = P[8 * h + jj];
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            if (pr == 0) {
+#pragma unroll
+                for (int jj = 0; jj < 8; ++jj) {
+                    const int j = 8 * h + jj;
+                    const int r = wm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                    const double P1 = X[((0 * 4 + blk) * 8 + jj) * 64 + lane];
+                    const double P2 = X[((1 * 4 + blk) * 8 + jj) * 64 + lane];
+                    const d2v bv = {P[j] - P1, P2 - P[j] - P1};
+                    __builtin_nontemporal_store(bv,
+                                                reinterpret_cast<d2v *>(&a.bmat[(size_t)(tm * kBM + r) * a.ldn + col]));
+                }
+            }
+            // the round's values are read before the next round (or the
+            // next tile's loads, and the scale buffer's refill) overwrite
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         }
-        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
-    // the last tile's loads for a next tile may still be in flight: they
-    // write LDS, which must not be released (to another workgroup) before
-    // they land
+    // a last tile's loads for a next tile may still be in flight: they
+    // write LDS, which must not be released before they land
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // one workgroup per CU, each walking its share of the tiles
-__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_kernel(GemmArgs a, int tiles)
+__global__ __launch_bounds__(768, 1) void gpp_oz_gemm_kernel(GemmArgs a, int tiles)
 {
     const int first = blockIdx.x, stride = gridDim.x;
     if (first >= tiles) return;
-    oz_gemm_tiles<true>(a, first, stride, (tiles - 1 - first) / stride + 1);
+    oz_gemm12_tiles<true>(a, first, stride, (tiles - 1 - first) / stride + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -929,7 +926,7 @@ __global__ __launch_bounds__(256) void oz_prep_batch_kernel(const OzRecord *rec,
     oz_prep_block(pr, b - first[p]);
 }
 
-__global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecord *rec, const int *first, int n)
+__global__ __launch_bounds__(768, 1) void gpp_oz_gemm_batch_kernel(const OzRecord *rec, const int *first, int n)
 {
     const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
     const int local = b - first[p];
@@ -937,7 +934,7 @@ __global__ __launch_bounds__(256, 1) void gpp_oz_gemm_batch_kernel(const OzRecor
     // a copy in registers: the tile's direct loads are asm statements that
     // clobber memory, after which fields read through rec would be reloaded
     const GemmArgs ga = rec[p].ga;
-    oz_gemm_tiles<false>(ga, local, 1, 1);
+    oz_gemm12_tiles<false>(ga, local, 1, 1);
 }
 
 __global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
@@ -997,7 +994,7 @@ int gpp_oz_batch_launch(const void *records, const int *first_prep, const int *f
     if (total_prep > 0)
         hipLaunchKernelGGL(oz_prep_batch_kernel, dim3(total_prep), dim3(256), 0, s, rec, first_prep, n);
     if (total_gemm > 0)
-        hipLaunchKernelGGL(gpp_oz_gemm_batch_kernel, dim3(total_gemm), dim3(256), 0, s, rec, first_gemm, n);
+        hipLaunchKernelGGL(gpp_oz_gemm_batch_kernel, dim3(total_gemm), dim3(kWaves12 * 64), 0, s, rec, first_gemm, n);
     if (total_pole > 0)
         hipLaunchKernelGGL(gpp_oz_pole_batch_kernel, dim3(total_pole), dim3(256), 0, s, rec, first_pole, n);
     FK_HIP(hipGetLastError());
@@ -1060,7 +1057,8 @@ int gpp_oz_launch(const fk_gpp_problem *p, const fk_options *o, const OzLayout &
     int dev = 0, ncu = 0;
     FK_HIP(hipGetDevice(&dev));
     FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(std::min(tiles, std::max(ncu, 1))), dim3(256), 0, s, ga, tiles);
+    hipLaunchKernelGGL(gpp_oz_gemm_kernel, dim3(std::min(tiles, std::max(ncu, 1))), dim3(kWaves12 * 64), 0, s, ga,
+                       tiles);
     FK_HIP(hipGetLastError());
     FK_TRY(timer.stop());
 
