@@ -300,8 +300,9 @@ int fk_instance_list_read(const char *path, char ***paths, int32_t *count)
     const size_t slash = dir.find_last_of('/');
     dir = slash == std::string::npos ? std::string() : dir.substr(0, slash + 1);
     std::vector<std::string> items;
-    char line[4096];
-    while (std::fgets(line, sizeof(line), f)) {
+    char *line = nullptr;       // getline grows it: a path of any length stays one entry
+    size_t cap = 0;
+    while (::getline(&line, &cap, f) >= 0) {
         std::string s(line);
         while (!s.empty() && (s.back() == '\n' || s.back() == '\r' || s.back() == ' ' || s.back() == '\t')) s.pop_back();
         size_t b = 0;
@@ -310,6 +311,7 @@ int fk_instance_list_read(const char *path, char ***paths, int32_t *count)
         if (s.empty() || s[0] == '#') continue;
         items.push_back(s[0] == '/' ? s : dir + s);
     }
+    std::free(line);
     std::fclose(f);
     char **arr = static_cast<char **>(std::calloc(items.size() + 1, sizeof(char *)));
     if (!arr) { set_error("fk_instance_list_read: out of memory"); return FK_ERR_NOMEM; }

@@ -110,6 +110,11 @@ def test_instance_list(fk, tmp_path):
     empty = sub / "empty.txt"
     empty.write_text("# nothing\n")
     assert fk.instance_list_read(str(empty)) == []
+    # a path longer than any fixed line buffer stays one entry; no final newline
+    deep = "/".join(["d" * 200] * 30) + "/p.fki"
+    lng = sub / "long.txt"
+    lng.write_text(f"{deep}\nlast.fki")
+    assert fk.instance_list_read(str(lng)) == [str(sub / deep), str(sub / "last.fki")]
 
 
 def test_header_whose_extents_wrap_is_rejected(fk, tmp_path):
