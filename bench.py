@@ -410,14 +410,24 @@ def run_batched(fk, torch, steps, singles=False):
             # each problem alone through gpp_kernel, best tile configuration
             # per problem, fused-kernel time (kernel_ms)
             tot = 0.0
+            per_cfg = {c: 0.0 for c in (1, 2, 3, 4)}
+            wins = {c: 0 for c in (1, 2, 3, 4)}
             for p in probs:
-                best = min(min(fk.gpp_kernel(p["nb"], p["nv"], p["ncouls"], p["ngpown"], p["wx"], p["aqsm"],
-                                             p["aqsn"], p["I_eps"], p["wtilde"], p["vcoul"], p["inv_igp_index"],
-                                             p["indinv"], config=c, time_kernel=True)[2] for _ in range(3))
-                           for c in (1, 2, 3, 4))
-                tot += best
+                t = {c: min(fk.gpp_kernel(p["nb"], p["nv"], p["ncouls"], p["ngpown"], p["wx"], p["aqsm"],
+                                          p["aqsn"], p["I_eps"], p["wtilde"], p["vcoul"], p["inv_igp_index"],
+                                          p["indinv"], config=c, time_kernel=True)[2] for _ in range(3))
+                     for c in (1, 2, 3, 4)}
+                c_best = min(t, key=t.get)
+                tot += t[c_best]
+                wins[c_best] += 1
+                for c in t:
+                    per_cfg[c] += t[c]
             result[prec]["sum_of_single_problem_kernel_ms"] = tot
-            log(f"batched {prec}: the 64 problems one by one, best configuration each: {tot:.2f} ms of kernels")
+            result[prec]["single_problem_kernel_ms_per_config"] = per_cfg
+            result[prec]["problems_fastest_at_config"] = wins
+            log(f"batched {prec}: the 64 problems one by one, best configuration each: {tot:.2f} ms of kernels "
+                f"(per configuration {', '.join(f'{c}: {v:.2f}' for c, v in per_cfg.items())}; "
+                f"fastest at {', '.join(f'{c}: {n}' for c, n in wins.items())} problems)")
         if prec == "fp64":
             fl = sum(gpp_alg_flops(nb, nv, C, G, nw) for nb, nv, C, G, nw in shapes)
             result[prec]["achieved_tflops"] = fl / (ms * 1e-3) / 1e12

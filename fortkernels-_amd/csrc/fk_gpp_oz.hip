@@ -620,38 +620,52 @@ __device__ __forceinline__ double block_sum(const double (&v)[V], PoleLds &L)
 
 // The W frequencies wq[0..W-1] of one element: the per-frequency steps of
 // pole_element's hoisted form, every frequency live.
+//
+// The fast path forms delw with shorter numerators and |delw|^2 as
+// |wtilde|^2 / |wdiff|^2 (equal to the reference's delw.x^2 + delw.y^2 up to
+// rounding, a few 1e-15 relative with the one-step reciprocal).  Every point
+// whose |delw|^2 falls within 1e-13 of a branch threshold is evaluated again
+// in the reference's order with the IEEE quotient, so every branch decision
+// is the reference's; |wdiff|^2, which the mask compares with limittwo, is
+// always the reference's (contraction off).  Both conditional blocks are
+// plain divergent branches, skipped when no lane of the wave takes them (a
+// ballot in front of them adds two vector instructions per frequency).
 template <int W>
 __device__ __forceinline__ void pole_freqs(const PoleParams &pp, const double (&wq)[W], const double (&iq)[W],
                                            double2 wt, double2 Bie, double2 BoOm, double2 Boie, bool want_ssx,
                                            double2 (&sach)[W], double2 (&sasx)[W])
 {
     const double tol1 = 1e-13 * pp.limitone, tol2 = 1e-13 * pp.to1;
+    double wy2;
+    {
+#pragma clang fp contract(off)
+        wy2 = wt.y * wt.y;                   // wdi^2, as the reference rounds it
+    }
+    const double wt2 = __builtin_fma(wt.x, wt.x, wy2);
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         const double wdr = wq[q] - wt.x;
-        const double wdiffr = pole_wdiffr(wdr, -wt.y);
-        double2 delw;
-        double delwr;
-        pole_delw(wt, wdr, rcp_fast<1>(wdiffr), delw, delwr);
-        // points within 1e-13 of a branch threshold take the IEEE quotient,
-        // so every branch decision is the reference's
-        const bool near = fabs(delwr - pp.limitone) <= tol1 || fabs(delwr - pp.to1) <= tol2;
-        if (__ballot(near)) {
-            if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
+        double wdiffr;
+        {
+#pragma clang fp contract(off)
+            wdiffr = wdr * wdr + wy2;        // = pole_wdiffr(wdr, -wt.y)
         }
+        const double rden = rcp_fast<1>(wdiffr);
+        // numerators wt.x wdr - wdi^2 and wt.y (wt.x + wdr) = wt.y wx
+        double2 delw = make_double2(__builtin_fma(wt.x, wdr, -wy2) * rden, wt.y * wq[q] * rden);
+        double delwr = wt2 * rden;
+        const bool near = (int)(fabs(delwr - pp.limitone) <= tol1) | (int)(fabs(delwr - pp.to1) <= tol2);
+        if (near) pole_delw(wt, wdr, 1.0 / wdiffr, delw, delwr);
         const bool m = (wdiffr > pp.limittwo) && (delwr < pp.limitone);
         const double2 t = (m || !pp.mask_on) ? cmul(Bie, delw) : make_double2(0.0, 0.0);
         sach[q] = cadd(sach[q], t);
         if (want_ssx) {
             double2 S = m ? make_double2(BoOm.x * iq[q], BoOm.y * iq[q]) : make_double2(0.0, 0.0);
-            const bool need2 = !m && delwr > pp.to1;
-            if (__ballot(need2)) {
-                if (need2) {
-                    const double2 z = make_double2(delw.x + 0.5, delw.y);
-                    const double sc = -0.25 * rcp_fast<1>(z.x * z.x + z.y * z.y);
-                    const double2 v = cmul(Boie, cmul(delw, cconj(z)));
-                    S = make_double2(sc * v.x, sc * v.y);
-                }
+            if (!m && delwr > pp.to1) {
+                const double2 z = make_double2(delw.x + 0.5, delw.y);
+                const double sc = -0.25 * rcp_fast<1>(z.x * z.x + z.y * z.y);
+                const double2 v = cmul(Boie, cmul(delw, cconj(z)));
+                S = make_double2(sc * v.x, sc * v.y);
             }
             sasx[q] = cadd(sasx[q], S);
         }
@@ -874,14 +888,15 @@ __device__ __forceinline__ void oz_pole_block(const PoleArgs &a, int b)
     }
 }
 
-#ifdef FK_POLE_WAVES
-#define FK_POLE_ATTR __attribute__((amdgpu_waves_per_eu(FK_POLE_WAVES, FK_POLE_WAVES)))
-#else
-#define FK_POLE_ATTR
-#endif
+// Occupancy the pole kernels are held to: 5 waves per SIMD (<= 96 VGPRs)
+// for the stream form, 4 (<= 128) for a kernel with the blocks form.  The
+// forms fit without spilling; left free, the compiler takes a few registers
+// more and loses a wave.
+constexpr int pole_waves(int forms) { return forms == kPoleStream ? 5 : 4; }
 
 template <bool ACHS, int FORMS>
-__global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_kernel(PoleArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pole_waves(FORMS))))
+void gpp_oz_pole_kernel(PoleArgs a)
 {
     oz_pole_block<ACHS, FORMS>(a, blockIdx.x);
 }
@@ -895,10 +910,11 @@ inline int pole_chunks(int C) { return std::min(FK_POLE_CHUNKS, div_up(C, kPoleC
 // ---------------------------------------------------------------------------
 // Batches: one launch of each kernel above for many problems.  Every
 // problem has a record of its three kernels' arguments; block b of a batched
-// launch belongs to the problem p with first[p] <= b < first[p + 1] and runs
-// that problem's block b - first[p].  A problem's GEMM range starts at a
-// multiple of the XCD count, so its tile order keeps its XCD locality, and
-// the blocks padding a range return before any barrier.
+// prep or pole launch belongs to the problem p with first[p] <= b <
+// first[p + 1] and runs that problem's block b - first[p].  The GEMM launch
+// is persistent and walks the problems in turn; a problem's GEMM range
+// starts at a multiple of the XCD count, so its tile order keeps its XCD
+// locality.
 // ---------------------------------------------------------------------------
 struct OzRecord {
     PrepArgs pr;
@@ -926,18 +942,31 @@ __global__ __launch_bounds__(256) void oz_prep_batch_kernel(const OzRecord *rec,
     oz_prep_block(pr, b - first[p]);
 }
 
+// The GEMM of every problem, persistent: one workgroup per CU walks the
+// problems in order and takes, of problem p, the tiles f, f + grid, ... with
+// f = (w - first[p]) mod grid.  The rotation continues from problem to
+// problem (first[p] is the problem's running tile offset, a multiple of the
+// XCD count, so a tile runs on the XCD the single-problem kernel gives it),
+// and within a problem the next tile's first steps load during the current
+// tile's last two, as in gpp_oz_gemm_kernel.  Between problems a workgroup
+// needs no barrier: each wave's LDS slots are its own and every wave drains
+// its loads and passes the epilogue's last barrier before returning.
 __global__ __launch_bounds__(768, 1) void gpp_oz_gemm_batch_kernel(const OzRecord *rec, const int *first, int n)
 {
-    const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
-    const int local = b - first[p];
-    if (local >= rec[p].gemm_tiles) return;
-    // a copy in registers: the tile's direct loads are asm statements that
-    // clobber memory, after which fields read through rec would be reloaded
-    const GemmArgs ga = rec[p].ga;
-    oz_gemm12_tiles<false>(ga, local, 1, 1);
+    const int w = (int)blockIdx.x, grid = (int)gridDim.x;
+    for (int p = 0; p < n; ++p) {
+        const int tiles = __builtin_amdgcn_readfirstlane(rec[p].gemm_tiles);
+        const int f = (w - __builtin_amdgcn_readfirstlane(first[p]) % grid + grid) % grid;
+        if (f >= tiles) continue;
+        // a copy in registers: the tile's direct loads are asm statements that
+        // clobber memory, after which fields read through rec would be reloaded
+        const GemmArgs ga = rec[p].ga;
+        oz_gemm12_tiles<true>(ga, f, grid, (tiles - 1 - f) / grid + 1);
+    }
 }
 
-__global__ __launch_bounds__(256) FK_POLE_ATTR void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pole_waves(kPoleStream | kPoleBlocks))))
+void gpp_oz_pole_batch_kernel(const OzRecord *rec, const int *first, int n)
 {
     const int b = (int)blockIdx.x, p = batch_problem(first, n, b);
     // by reference: a register copy of the arguments (with the pole
@@ -993,8 +1022,13 @@ int gpp_oz_batch_launch(const void *records, const int *first_prep, const int *f
     const OzRecord *rec = static_cast<const OzRecord *>(records);
     if (total_prep > 0)
         hipLaunchKernelGGL(oz_prep_batch_kernel, dim3(total_prep), dim3(256), 0, s, rec, first_prep, n);
-    if (total_gemm > 0)
-        hipLaunchKernelGGL(gpp_oz_gemm_batch_kernel, dim3(total_gemm), dim3(kWaves12 * 64), 0, s, rec, first_gemm, n);
+    if (total_gemm > 0) {
+        int dev = 0, ncu = 0;
+        FK_HIP(hipGetDevice(&dev));
+        FK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int grid = std::max(1, std::min(total_gemm, ncu));
+        hipLaunchKernelGGL(gpp_oz_gemm_batch_kernel, dim3(grid), dim3(kWaves12 * 64), 0, s, rec, first_gemm, n);
+    }
     if (total_pole > 0)
         hipLaunchKernelGGL(gpp_oz_pole_batch_kernel, dim3(total_pole), dim3(256), 0, s, rec, first_pole, n);
     FK_HIP(hipGetLastError());
