@@ -1117,8 +1117,18 @@ struct GppBatch {
             a.igp = reinterpret_cast<const int *>(at(b.off_igp));
             a.partials = reinterpret_cast<double2 *>(at(b.off_part));
             a.nb = b.nb; a.nv = b.nv; a.C = b.C; a.G = b.G; a.nw = b.nw;
-            a.ldc = b.ldc; a.Kpad = b.Kpad; a.nvpad = b.nvpad; a.nks = b.Kpad / 4;
-            a.tiles_m = b.ldc / t.bm;
+            a.ldc = b.ldc; a.nvpad = b.nvpad;
+            // The workspace is padded for every configuration (bands to 48,
+            // rows to 128); the tiles walk only this configuration's own
+            // padding, as a single call of it does (bands to 4 x ring depth,
+            // rows to the tile height).  The coefficient rows past the bands
+            // are zero and rows past G are masked, so reads stay inside the
+            // larger padding.  (With the common padding the batch ran 3.5%
+            // slower than the single calls.)
+            const int kstep = 4 * t.p;
+            a.Kpad = div_up(std::max(b.nvpad + b.nb - b.nv, 1), kstep) * kstep;
+            a.nks = a.Kpad / 4;
+            a.tiles_m = div_up(b.G, t.bm);
             a.tiles_n = div_up(b.C, t.bn);
             a.group_m = a.tiles_m;   // unused: the batch walks its own tile table
             a.stagger_lo = a.stagger_hi = a.stagger_ticks = 0;
