@@ -660,14 +660,18 @@ __device__ __forceinline__ void pole_freqs(const PoleParams &pp, const double (&
         const double2 t = (m || !pp.mask_on) ? cmul(Bie, delw) : make_double2(0.0, 0.0);
         sach[q] = cadd(sach[q], t);
         if (want_ssx) {
-            double2 S = m ? make_double2(BoOm.x * iq[q], BoOm.y * iq[q]) : make_double2(0.0, 0.0);
+            // mask branch: Bo Omega2 / wx^2, added through a selected factor
+            // (Bo Omega2 is finite, so a zero factor adds an exact zero)
+            const double f = m ? iq[q] : 0.0;
+            sasx[q].x = __builtin_fma(BoOm.x, f, sasx[q].x);
+            sasx[q].y = __builtin_fma(BoOm.y, f, sasx[q].y);
             if (!m && delwr > pp.to1) {
                 const double2 z = make_double2(delw.x + 0.5, delw.y);
                 const double sc = -0.25 * rcp_fast<1>(z.x * z.x + z.y * z.y);
                 const double2 v = cmul(Boie, cmul(delw, cconj(z)));
-                S = make_double2(sc * v.x, sc * v.y);
+                sasx[q].x = __builtin_fma(sc, v.x, sasx[q].x);
+                sasx[q].y = __builtin_fma(sc, v.y, sasx[q].y);
             }
-            sasx[q] = cadd(sasx[q], S);
         }
     }
 }
