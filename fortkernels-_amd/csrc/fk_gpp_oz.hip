@@ -893,9 +893,10 @@ __device__ __forceinline__ void oz_pole_block(const PoleArgs &a, int b)
 }
 
 // Occupancy the pole kernels are held to: 5 waves per SIMD (<= 96 VGPRs)
-// for the stream form, 4 (<= 128) for a kernel with the blocks form.  The
-// forms fit without spilling; left free, the compiler takes a few registers
-// more and loses a wave.
+// for the stream form, 4 (<= 128) for a kernel with the blocks form.  To fit,
+// the compiler spills a few registers (1 to 12 a lane) in the set-up and at
+// block boundaries, outside the frequency loop; left free, it takes a few
+// registers more and loses a wave, which measured slower.
 constexpr int pole_waves(int forms) { return forms == kPoleStream ? 5 : 4; }
 
 template <bool ACHS, int FORMS>
